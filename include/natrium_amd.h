@@ -1,0 +1,128 @@
+/* natrium_amd: MI355X (gfx950) time-stepping hot path of a high-order
+ * semi-Lagrangian / SEDG lattice Boltzmann solver, behind a plain C ABI.
+ *
+ * Every entry point returns 0 on success or a NAT_ERR_* code; the message of the
+ * last failure on a context is available from nat_last_error(). No exception and
+ * no C++ type crosses this boundary. Populations and operators stay in device
+ * memory between calls; each context owns one HIP stream and calls on different
+ * contexts may run from different threads. Calls on one context must not overlap.
+ *
+ * Array conventions (host pointers unless the name says device):
+ *   DoF numbering   SL:   lattice nodes (N_a p + 1 per axis), x fastest; nodes on
+ *                         opposite periodic faces are distinct DoFs, as in the
+ *                         reference's unconstrained FE_Q space.
+ *                   SEDG: cell-major, cell c (x fastest) then local GLL node
+ *                         (x fastest), (p+1)^D nodes per cell.
+ *   populations     f[alpha * n_dofs + i], alpha = 0..Q-1 (structure of arrays).
+ *   velocities      u[d * n_dofs + i], in physical units (u_lattice * scaling).
+ *   vertices        (N_0+1)(N_1+1)[(N_2+1)] points, x fastest, dim doubles each.
+ */
+#ifndef NATRIUM_AMD_H
+#define NATRIUM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* stencils (stencils/D2Q9.cpp, D3Q19.cpp, D3Q27.cpp, D2Q25H.cpp of the reference) */
+enum { NAT_D2Q9 = 0, NAT_D3Q19 = 1, NAT_D3Q27 = 2, NAT_D2Q25H = 3 };
+/* advection schemes (solver/SolverConfiguration.cpp:45 "Advection scheme") */
+enum { NAT_SL = 0, NAT_SEDG = 1, NAT_SEDG_CENTRAL = 2 };
+/* arithmetic of the bulk populations */
+enum { NAT_FP64 = 0, NAT_FP32_SHIFTED = 1 };
+/* equilibria (collision_advanced/Equilibria.h) */
+enum { NAT_EQ_BGK = 0, NAT_EQ_QUARTIC = 1 };
+/* SL backend selection */
+enum { NAT_BACKEND_AUTO = 0, NAT_BACKEND_STRUCTURED = 1, NAT_BACKEND_GENERAL = 2 };
+
+enum {
+  NAT_OK = 0,
+  NAT_ERR_ARG = 1,          /* invalid argument / descriptor */
+  NAT_ERR_UNSUPPORTED = 2,  /* combination not supported by this build */
+  NAT_ERR_HIP = 3,          /* HIP runtime failure */
+  NAT_ERR_DENSITY = 4,      /* density < 1e-10 in collision (DensityZeroException) */
+  NAT_ERR_STATE = 5,        /* call out of order (e.g. stepping before set_dt) */
+  NAT_ERR_NOMEM = 6         /* device allocation failed */
+};
+
+typedef struct nat_ctx nat_ctx;
+
+typedef struct nat_desc {
+  int dim;               /* 2 or 3 */
+  int stencil;           /* NAT_D2Q9 ... */
+  int order;             /* FE order p >= 1 */
+  int ncells[3];         /* cells per axis */
+  const double* vertices;/* lattice vertices, see conventions above */
+  int cartesian;         /* 1: axis-aligned cells (MappingCartesian); 0: trilinear map */
+  int periodic[3];       /* periodic flag per axis (walls are not part of this build) */
+  double scaling;        /* stencil scaling s (|c_axis| = s) */
+  double viscosity;      /* kinematic viscosity nu */
+  double cfl;            /* CFL number; dt = CFL h_min / (sqrt(2) s p^2) */
+  int scheme;            /* NAT_SL, NAT_SEDG or NAT_SEDG_CENTRAL */
+  int precision;         /* NAT_FP64 or NAT_FP32_SHIFTED */
+  int equilibrium;       /* NAT_EQ_BGK or NAT_EQ_QUARTIC */
+  int compressible;      /* 1: f and g populations, quartic equilibrium, FP64 temperature */
+  double gamma;          /* heat capacity ratio (compressible) */
+  double rho0;           /* reference density of the FP32 shift f - w rho0 (0 -> 1.0) */
+  int backend;           /* NAT_BACKEND_* (SL only) */
+} nat_desc;
+
+/* Context lifetime. Replaces the CFDSolver<dim> constructor's set-up of the
+ * stencil, DoF handler and distribution vectors (solver/CFDSolver.cpp:77-255). */
+int nat_create(const nat_desc* desc, nat_ctx** out);
+void nat_destroy(nat_ctx* ctx);
+const char* nat_last_error(const nat_ctx* ctx);
+
+int64_t nat_n_dofs(const nat_ctx* ctx);
+int nat_n_stored_nodes(const nat_ctx* ctx, int64_t* n); /* nodes held on the device */
+int nat_Q(const nat_ctx* ctx);
+double nat_dt(const nat_ctx* ctx);     /* CFDSolverUtilities::calculateTimestep (CFDSolverUtilities.cpp:92-99) */
+double nat_tau(const nat_ctx* ctx);    /* calculateTauFromNu (AuxiliaryCollisionFunctions.h:63-68) */
+double nat_time(const nat_ctx* ctx);
+int nat_backend(const nat_ctx* ctx);   /* NAT_BACKEND_STRUCTURED or NAT_BACKEND_GENERAL for SL */
+int nat_node_coords(nat_ctx* ctx, double* xyz /* n_dofs x 3 */);
+
+/* AdvectionOperator::setDeltaT + reassemble (SemiLagrangian.h:175-179,
+ * SemiLagrangian.cpp:76-90; SEDGMinLee.cpp:92-184). Departure points and basis
+ * weights are located on the GPU in FP64. dt <= 0 keeps the CFL time step. */
+int nat_set_dt(nat_ctx* ctx, double dt);
+
+/* CFDSolver::initializeDistributions (CFDSolver.cpp:1104-1129; compressible:
+ * CompressibleCFDSolver.h:850-946): f = equilibrium(rho, u[, T]); T may be NULL. */
+int nat_init_equilibrium(nat_ctx* ctx, const double* rho, const double* u, const double* T);
+
+/* DistributionFunctions copies (solver/DistributionFunctions.h:47-300): FP64 host arrays. */
+int nat_upload_f(nat_ctx* ctx, const double* f);
+int nat_download_f(nat_ctx* ctx, double* f);
+int nat_upload_g(nat_ctx* ctx, const double* g);
+int nat_download_g(nat_ctx* ctx, double* g);
+
+/* CFDSolver::collide (CFDSolver.cpp:807-843 -> selectCollision ->
+ * CollisionOperator::collideAll, CollisionOperator.h:26-224). Writes rho, u (and T). */
+int nat_collide(nat_ctx* ctx);
+
+/* CFDSolver::stream (CFDSolver.cpp:659-754): SL operator apply, or one low-storage
+ * RK5 step of the SEDG operator (RungeKutta5LowStorage.cpp:56-105). */
+int nat_stream(nat_ctx* ctx);
+
+/* nsteps iterations of the CFDSolver::run loop body (CFDSolver.cpp:884-888):
+ * stream(); collide(); with stream and collide fused on the device. The caller
+ * runs the initial collide() of run() once with nat_collide(). */
+int nat_step(nat_ctx* ctx, int nsteps);
+
+/* macroscopic fields written by the last collision (FP64 host arrays; T may be NULL) */
+int nat_get_macro(nat_ctx* ctx, double* rho, double* u, double* T);
+
+/* PhysicalProperties::mass / kineticEnergy (PhysicalProperties.cpp:29-130, 270-310),
+ * plus total momentum, reduced on the device in FP64 from the last collision's fields. */
+int nat_sums(nat_ctx* ctx, double* mass, double* momentum /* dim */, double* ekin);
+
+int nat_sync(nat_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NATRIUM_AMD_H */

@@ -1,0 +1,72 @@
+// Pointwise BGK collision in registers, shared by the fused step kernels.
+//
+// Semantics of collision_advanced/CollisionOperator.h:52-102 with
+// BGKCollision::relax (CollisionSchemes.h:28-41) and BGKEquilibrium
+// (Equilibria.h:66-83): rho = sum f, u = sum e f / rho (unscaled e), relax
+// f -= (f - f_eq)/tau, write rho and u*scaling.
+//
+// Populations may be stored shifted, f~ = f - w rho0 (rho0 = 0: unshifted FP64).
+// Because sum_i w_i = 1 and sum_i w_i e_i = 0, rho = rho0 + sum f~ and
+// rho u = sum e f~ exactly, and f~_eq = w (drho + rho (ue (1 + ue/2) - u.u/(2 cs2))),
+// so BGK relaxes f~ without ever forming the O(1) background.
+#pragma once
+#include "nat_internal.hpp"
+
+namespace nat {
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+// cs2 = 1/3 in unscaled units for D2Q9 / D3Q19 / D3Q27
+template <int Q, typename T>
+__device__ __forceinline__ void bgk_collide(T (&f)[Q], const CollideArgs<T>& ca, int64_t node, bool write_macro) {
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  T drho = 0;
+#pragma unroll
+  for (int i = 0; i < Q; i++) drho += f[i];
+  T j[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (L::e[i][d] > 0) j[d] += f[i];
+      if (L::e[i][d] < 0) j[d] -= f[i];
+    }
+  }
+  const T rho = ca.rho0 + drho;
+  if (!(rho >= T(1e-10))) atomicOr(ca.status, 1);
+  const T inv = T(1) / rho;
+  T u[3];
+#pragma unroll
+  for (int d = 0; d < D; d++) u[d] = j[d] * inv;
+  T uu = 0;
+#pragma unroll
+  for (int d = 0; d < D; d++) uu += u[d] * u[d];
+  const T c_uu = T(1.5) * uu;
+  const T om = ca.omega;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    T ue = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (L::e[i][d] > 0) ue += u[d];
+      if (L::e[i][d] < 0) ue -= u[d];
+    }
+    ue *= T(3);
+    const T feq = T(L::w[i]) * (drho + rho * (ue * (T(1) + T(0.5) * ue) - c_uu));
+    f[i] -= om * (f[i] - feq);
+  }
+  if (write_macro && ca.rho) {
+    ca.rho[node] = rho;
+#pragma unroll
+    for (int d = 0; d < D; d++) ca.u[d * ca.mstride + node] = u[d] * T(ca.scaling);
+  }
+}
+
+}  // namespace nat

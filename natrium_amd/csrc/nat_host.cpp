@@ -1,0 +1,768 @@
+// C-ABI layer of natrium_amd: context set-up, operator assembly dispatch,
+// population upload/download and the step loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "nat_internal.hpp"
+
+// kernel launchers (k_*.hip)
+int nat_launch_assemble_general(nat_ctx* ctx, const double* d_disp, int* d_cell, double* d_xi);
+int nat_launch_assemble_axis(nat_ctx* ctx, const int* d_N, const double* d_vaxis, int maxc, const double* d_disp,
+                             const int* d_nshift, double* d_w, int* d_off);
+int nat_launch_xi_to_f32(nat_ctx* ctx, const double* src, float* dst, int64_t n);
+int nat_launch_sl_struct(nat_ctx* ctx, int write_macro);
+int nat_launch_sl_general(nat_ctx* ctx, int write_macro);
+int nat_launch_collide(nat_ctx* ctx);
+int nat_launch_sums(nat_ctx* ctx, int layout);
+int nat_sedg_setup(nat_ctx* ctx);
+int nat_sedg_step(nat_ctx* ctx, int collide_after, int write_macro);
+
+namespace nat {
+
+bool make_stencil(int id, double s, StencilHost& st) {
+  st = StencilHost();
+  st.id = id;
+  st.s = s;
+  auto put = [&](int i, double x, double y, double z) {
+    st.dir[i][0] = x;
+    st.dir[i][1] = y;
+    st.dir[i][2] = z;
+  };
+  if (id == NAT_D2Q9 || id == NAT_D3Q19 || id == NAT_D3Q27) {
+    int Q = id == NAT_D2Q9 ? 9 : (id == NAT_D3Q19 ? 19 : 27);
+    st.D = id == NAT_D2Q9 ? 2 : 3;
+    st.Q = Q;
+    for (int i = 0; i < Q; i++) {
+      const int* e = Q == 9 ? Lattice<9>::e[i] : (Q == 19 ? Lattice<19>::e[i] : Lattice<27>::e[i]);
+      put(i, e[0] * s, e[1] * s, e[2] * s);
+      st.w[i] = Q == 9 ? Lattice<9>::w[i] : (Q == 19 ? Lattice<19>::w[i] : Lattice<27>::w[i]);
+    }
+  } else if (id == NAT_D2Q25H) {
+    // stencils/D2Q25H.cpp:42-82: tensor product of the 5-velocity Hermite set
+    st.D = 2;
+    st.Q = 25;
+    const double r = (std::sqrt(5.) - std::sqrt(2.)) / std::sqrt(3.);
+    const double w0 = (-3 - 3 * r * r * r * r + 54 * r * r) / (75 * r * r);
+    const double wm = (9 * r * r * r * r - 6 - 27 * r * r) / (300 * r * r * (r * r - 1));
+    const double wn = (9 - 6 * r * r * r * r - 27 * r * r) / (300 * (1 - r * r));
+    const double cm = s * std::sqrt(5. - std::sqrt(10.)) / std::sqrt(3.);
+    const double cn = std::sqrt(5. + std::sqrt(10.)) * s / std::sqrt(3.);
+    const double c1[25][2] = {{0, 0},   {cm, 0},    {0, cm},   {-cm, 0},  {0, -cm},  {cm, cm},   {-cm, cm},
+                              {-cm, -cm}, {cm, -cm}, {cn, 0},   {0, cn},   {-cn, 0},  {0, -cn},   {cn, cn},
+                              {-cn, cn}, {-cn, -cn}, {cn, -cn}, {cm, cn},  {cm, -cn}, {-cm, -cn}, {-cm, cn},
+                              {cn, cm},  {cn, -cm}, {-cn, -cm}, {-cn, cm}};
+    const double ww[25] = {w0 * w0, w0 * wm, w0 * wm, w0 * wm, w0 * wm, wm * wm, wm * wm, wm * wm, wm * wm,
+                           w0 * wn, w0 * wn, w0 * wn, w0 * wn, wn * wn, wn * wn, wn * wn, wn * wn, wm * wn,
+                           wm * wn, wm * wn, wm * wn, wm * wn, wm * wn, wm * wn, wm * wn};
+    for (int i = 0; i < 25; i++) {
+      put(i, c1[i][0], c1[i][1], 0);
+      st.w[i] = ww[i];
+    }
+  } else {
+    return false;
+  }
+  st.cs2 = s * s / 3.;
+  st.max_speed = s * std::sqrt(2.);
+  return true;
+}
+
+void gll_points(int p, double* x, double* w) {
+  // Gauss-Lobatto on [-1,1]: endpoints and the roots of P_p', by Newton from
+  // the Chebyshev-Gauss-Lobatto points; mapped to [0,1].
+  const int n = p + 1;
+  std::vector<long double> xs(n), ws(n);
+  for (int j = 0; j < n; j++) {
+    long double t = -std::cos(3.14159265358979323846264338327950288L * j / p);
+    auto leg = [&](long double z, long double& pn, long double& pm) {
+      long double a = 1, b = z;
+      if (p == 0) {
+        pn = 1;
+        pm = 0;
+        return;
+      }
+      for (int k = 2; k <= p; k++) {
+        long double c = ((2 * k - 1) * z * b - (k - 1) * a) / k;
+        a = b;
+        b = c;
+      }
+      pn = b;
+      pm = a;
+    };
+    if (j == 0) t = -1;
+    if (j == p) t = 1;
+    if (j > 0 && j < p)
+      for (int it = 0; it < 100; it++) {
+        long double pn, pm;
+        leg(t, pn, pm);
+        long double dt = (t * pn - pm) / ((p + 1) * pn);
+        t -= dt;
+        if (std::fabs((double)dt) < 1e-19) break;
+      }
+    long double pn, pm;
+    leg(t, pn, pm);
+    xs[j] = t;
+    ws[j] = 2.0L / (p * (p + 1) * pn * pn);
+  }
+  for (int j = 0; j < n; j++) {
+    long double a = (xs[j] + 1) / 2, b = (1 - xs[p - j]) / 2;
+    x[j] = (double)((a + b) / 2);
+    w[j] = (double)(ws[j] / 2);
+  }
+  x[0] = 0.0;
+  x[p] = 1.0;
+  if (p % 2 == 0) x[p / 2] = 0.5;
+}
+
+}  // namespace nat
+
+using namespace nat;
+
+namespace {
+
+const double* vtx(const nat_ctx* c, int i, int j, int k) {
+  return &c->verts[3 * ((size_t)i + (size_t)(c->desc.ncells[0] + 1) * ((size_t)j + (size_t)(c->desc.ncells[1] + 1) * k))];
+}
+
+void cell_vertex(const nat_ctx* c, const int cc[3], int v, double out[3]) {
+  const double* p = vtx(c, cc[0] + (v & 1), cc[1] + ((v >> 1) & 1), c->D == 3 ? cc[2] + ((v >> 2) & 1) : 0);
+  out[0] = p[0];
+  out[1] = p[1];
+  out[2] = p[2];
+}
+
+void map_cell(const nat_ctx* c, const int cc[3], const double xi[3], bool cartesian, double x[3]) {
+  const int nv = c->D == 3 ? 8 : 4;
+  if (cartesian) {
+    double a[3], b[3];
+    cell_vertex(c, cc, 0, a);
+    cell_vertex(c, cc, nv - 1, b);
+    for (int d = 0; d < 3; d++) x[d] = d < c->D ? a[d] + xi[d] * (b[d] - a[d]) : 0.0;
+    return;
+  }
+  x[0] = x[1] = x[2] = 0;
+  for (int v = 0; v < nv; v++) {
+    double Nv = 1, X[3];
+    for (int a = 0; a < c->D; a++) Nv *= ((v >> a) & 1) ? xi[a] : (1 - xi[a]);
+    cell_vertex(c, cc, v, X);
+    for (int d = 0; d < c->D; d++) x[d] += Nv * X[d];
+  }
+}
+
+double min_vertex_distance(const nat_ctx* c) {
+  double mind = 1e300;
+  const int nv = c->D == 3 ? 8 : 4;
+  const int nz = c->D == 3 ? c->desc.ncells[2] : 1;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < c->desc.ncells[1]; j++)
+      for (int i = 0; i < c->desc.ncells[0]; i++) {
+        int cc[3] = {i, j, k};
+        double V[8][3];
+        for (int v = 0; v < nv; v++) cell_vertex(c, cc, v, V[v]);
+        for (int a = 0; a < nv; a++)
+          for (int b = a + 1; b < nv; b++) {
+            double s = 0;
+            for (int d = 0; d < c->D; d++) s += (V[a][d] - V[b][d]) * (V[a][d] - V[b][d]);
+            mind = std::min(mind, std::sqrt(s));
+          }
+      }
+  return mind;
+}
+
+// uniform axis-aligned lattice (a prerequisite of the structured backend)
+bool is_uniform_cartesian(const nat_ctx* c) {
+  const int D = c->D;
+  const int N[3] = {c->desc.ncells[0], c->desc.ncells[1], D == 3 ? c->desc.ncells[2] : 1};
+  for (int a = 0; a < D; a++) {
+    const double h0 = vtx(c, a == 0 ? 1 : 0, a == 1 ? 1 : 0, a == 2 ? 1 : 0)[a] - vtx(c, 0, 0, 0)[a];
+    if (!(h0 > 0)) return false;
+    for (int k = 0; k <= (D == 3 ? N[2] : 0); k++)
+      for (int j = 0; j <= N[1]; j++)
+        for (int i = 0; i <= N[0]; i++) {
+          const double* p = vtx(c, i, j, k);
+          const int idx[3] = {i, j, k};
+          // axis a coordinate depends only on idx[a], with spacing h0
+          const double xa = p[a];
+          const double ref = vtx(c, a == 0 ? i : 0, a == 1 ? j : 0, a == 2 ? k : 0)[a];
+          if (std::fabs(xa - ref) > 1e-12 * std::fabs(h0)) return false;
+          if (idx[a] > 0) {
+            const double prev = vtx(c, a == 0 ? i - 1 : 0, a == 1 ? j - 1 : 0, a == 2 ? k - 1 : 0)[a];
+            if (std::fabs((ref - prev) - h0) > 1e-10 * h0) return false;
+          }
+        }
+  }
+  return true;
+}
+
+template <typename T>
+int alloc(nat_ctx* ctx, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return NAT_ERR_NOMEM;
+  }
+  return 0;
+}
+
+int fail(nat_ctx* ctx, int code, const std::string& msg) {
+  ctx->err = msg;
+  return code;
+}
+
+#define TRY(x)              \
+  do {                      \
+    int _rc = (x);          \
+    if (_rc) return _rc;    \
+  } while (0)
+
+int check_status(nat_ctx* ctx) {
+  int st = 0;
+  NAT_HIP(hipMemcpyAsync(&st, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  if (st) NAT_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
+  if (st & 1) return fail(ctx, NAT_ERR_DENSITY, "Densities too small (< 1e-10) for collisions. Decrease time step size.");
+  if (st & 2) return fail(ctx, NAT_ERR_UNSUPPORTED, "departure path reached a non-periodic boundary (walls are not supported)");
+  if (st & 4) return fail(ctx, NAT_ERR_ARG, "departure point could not be located in its cell");
+  if (st & 16) return fail(ctx, NAT_ERR_ARG, "structured departure walk left the axis");
+  return 0;
+}
+
+// FE_Q lattice: owning cell and support point of every DoF (SemiLagrangian.cpp:214-229)
+void feq_points(const nat_ctx* c, std::vector<double>& xyz, std::vector<int>& owner) {
+  double gll[MAXP + 1], gw[MAXP + 1];
+  gll_points(c->p, gll, gw);
+  const int64_t n = c->n_dofs;
+  xyz.assign(3 * n, 0.0);
+  owner.assign(n, 0);
+  for (int k = 0; k < c->nfe[2]; k++)
+    for (int j = 0; j < c->nfe[1]; j++)
+      for (int i = 0; i < c->nfe[0]; i++) {
+        const int idx[3] = {i, j, k};
+        int cc[3] = {0, 0, 0}, ml[3] = {0, 0, 0};
+        for (int a = 0; a < c->D; a++) {
+          cc[a] = idx[a] == 0 ? 0 : (idx[a] - 1) / c->p;
+          ml[a] = idx[a] - cc[a] * c->p;
+        }
+        const double xi[3] = {gll[ml[0]], gll[ml[1]], c->D == 3 ? gll[ml[2]] : 0.0};
+        double x[3];
+        map_cell(c, cc, xi, c->desc.cartesian != 0, x);
+        const int64_t d = i + (int64_t)c->nfe[0] * (j + (int64_t)c->nfe[1] * k);
+        xyz[3 * d] = x[0];
+        xyz[3 * d + 1] = x[1];
+        xyz[3 * d + 2] = x[2];
+        owner[d] = cc[0] + c->desc.ncells[0] * (cc[1] + c->desc.ncells[1] * cc[2]);
+      }
+}
+
+// DG nodes: cell-major, local lexicographic
+void dg_points(const nat_ctx* c, std::vector<double>& xyz) {
+  double gll[MAXP + 1], gw[MAXP + 1];
+  gll_points(c->p, gll, gw);
+  const int np = c->p + 1, nl = c->D == 3 ? np * np * np : np * np;
+  xyz.assign(3 * c->n_dofs, 0.0);
+  const int nz = c->D == 3 ? c->desc.ncells[2] : 1;
+  int64_t t = 0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < c->desc.ncells[1]; j++)
+      for (int i = 0; i < c->desc.ncells[0]; i++) {
+        int cc[3] = {i, j, k};
+        for (int l = 0; l < nl; l++, t++) {
+          const double xi[3] = {gll[l % np], gll[(l / np) % np], c->D == 3 ? gll[l / (np * np)] : 0.0};
+          double x[3];
+          map_cell(c, cc, xi, false, x);
+          for (int d = 0; d < 3; d++) xyz[3 * t + d] = x[d];
+        }
+      }
+}
+
+// device storage index of FE_Q DoF (i,j,k)
+inline int64_t store_index(const nat_ctx* c, int64_t dof) {
+  if (c->backend != NAT_BACKEND_STRUCTURED || c->desc.scheme != NAT_SL) return dof;
+  const int i = (int)(dof % c->nfe[0]);
+  const int j = (int)((dof / c->nfe[0]) % c->nfe[1]);
+  const int k = (int)(dof / ((int64_t)c->nfe[0] * c->nfe[1]));
+  const int ii = i % c->nu[0], jj = j % c->nu[1], kk = k % c->nu[2];
+  return ii + (int64_t)c->nu[0] * (jj + (int64_t)c->nu[1] * kk);
+}
+
+// Structured 1-D tables on the GPU (FP64), then host checks: offsets within one
+// cell layer and no tensor-product weight the reference would drop (|w| < 1e-10).
+int setup_structured_tables(nat_ctx* ctx, bool* ok) {
+  *ok = false;
+  const int D = ctx->D, p = ctx->p;
+  // shift id 0: component +s (departure at -dt s), 1: component -s
+  double disp[3 * MAXSHIFT] = {};
+  int nshift[3] = {0, 0, 0};
+  for (int a = 0; a < D; a++) {
+    nshift[a] = 2;
+    disp[a * MAXSHIFT + 0] = ctx->st.s * (-ctx->dt);
+    disp[a * MAXSHIFT + 1] = -ctx->st.s * (-ctx->dt);
+  }
+  int maxc = 0;
+  for (int a = 0; a < D; a++) maxc = std::max(maxc, ctx->desc.ncells[a]);
+  std::vector<double> vaxis(3 * (maxc + 1), 0.0);
+  for (int a = 0; a < D; a++)
+    for (int i = 0; i <= ctx->desc.ncells[a]; i++)
+      vaxis[a * (maxc + 1) + i] = vtx(ctx, a == 0 ? i : 0, a == 1 ? i : 0, a == 2 ? i : 0)[a];
+  int N[3] = {ctx->desc.ncells[0], ctx->desc.ncells[1], D == 3 ? ctx->desc.ncells[2] : 1};
+  int *dN = nullptr, *dns = nullptr, *doff = nullptr;
+  double *dva = nullptr, *ddisp = nullptr, *dw = nullptr;
+  const size_t nw = 3 * MAXSHIFT * MAXP * (MAXP + 1), no = 3 * MAXSHIFT * MAXP;
+  TRY(alloc(ctx, &dN, 3));
+  TRY(alloc(ctx, &dns, 3));
+  TRY(alloc(ctx, &doff, no));
+  TRY(alloc(ctx, &dva, vaxis.size()));
+  TRY(alloc(ctx, &ddisp, 3 * MAXSHIFT));
+  TRY(alloc(ctx, &dw, nw));
+  NAT_HIP(hipMemcpyAsync(dN, N, sizeof(N), hipMemcpyHostToDevice, ctx->stream));
+  NAT_HIP(hipMemcpyAsync(dns, nshift, sizeof(nshift), hipMemcpyHostToDevice, ctx->stream));
+  NAT_HIP(hipMemcpyAsync(dva, vaxis.data(), vaxis.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  NAT_HIP(hipMemcpyAsync(ddisp, disp, sizeof(disp), hipMemcpyHostToDevice, ctx->stream));
+  NAT_HIP(hipMemsetAsync(dw, 0, nw * sizeof(double), ctx->stream));
+  NAT_HIP(hipMemsetAsync(doff, 0, no * sizeof(int), ctx->stream));
+  TRY(nat_launch_assemble_axis(ctx, dN, dva, maxc, ddisp, dns, dw, doff));
+  std::vector<double> w(nw);
+  std::vector<int> off(no);
+  NAT_HIP(hipMemcpyAsync(w.data(), dw, nw * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  NAT_HIP(hipMemcpyAsync(off.data(), doff, no * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  hipFree(dN);
+  hipFree(dns);
+  hipFree(doff);
+  hipFree(dva);
+  hipFree(ddisp);
+  hipFree(dw);
+  int st = 0;
+  NAT_HIP(hipMemcpy(&st, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost));
+  if (st) {
+    NAT_HIP(hipMemset(ctx->d_status, 0, sizeof(int)));
+    return 0;
+  }
+  auto W = [&](int a, int s, int j, int m) { return w[((a * MAXSHIFT + s) * MAXP + j) * (MAXP + 1) + m]; };
+  for (int a = 0; a < D; a++)
+    for (int s = 0; s < 2; s++) {
+      int lo = 100, hi = -100;
+      for (int j = 0; j < p; j++) {
+        const int o = off[(a * MAXSHIFT + s) * MAXP + j];
+        lo = std::min(lo, o);
+        hi = std::max(hi, o);
+      }
+      if (lo < -1 || hi > 1 || hi - lo > 1) return 0;
+      ctx->off_lo[a][s] = lo;
+      ctx->off_hi[a][s] = hi;
+      // the kernel's fixed window pattern: for a positive component only the
+      // vertex node (j = 0) reads the cell below; otherwise every node reads its own cell
+      for (int j = 0; j < p; j++) {
+        const int o = off[(a * MAXSHIFT + s) * MAXP + j];
+        const int want = (s == 0 && j == 0) ? -1 : 0;
+        if (o != want) return 0;
+      }
+    }
+  // cut-off check over all directions and node classes
+  const int np = p + 1;
+  for (int al = 1; al < ctx->Q; al++) {
+    int sid[3] = {-1, -1, -1};
+    for (int a = 0; a < D; a++) {
+      const double c = ctx->st.dir[al][a];
+      sid[a] = c > 0 ? 0 : (c < 0 ? 1 : -1);
+    }
+    const int jz_n = D == 3 ? p : 1, mz_n = D == 3 ? np : 1;
+    for (int jz = 0; jz < jz_n; jz++)
+      for (int jy = 0; jy < p; jy++)
+        for (int jx = 0; jx < p; jx++)
+          for (int mz = 0; mz < mz_n; mz++)
+            for (int my = 0; my < np; my++)
+              for (int mx = 0; mx < np; mx++) {
+                const int jj[3] = {jx, jy, jz}, mm[3] = {mx, my, mz};
+                double prod = 1;
+                bool zero = false;
+                for (int a = 0; a < D; a++) {
+                  double v;
+                  if (sid[a] < 0) {
+                    if (mm[a] != jj[a]) zero = true;
+                    v = 1.0;
+                  } else {
+                    v = W(a, sid[a], jj[a], mm[a]);
+                  }
+                  prod *= v;
+                }
+                if (zero || prod == 0) continue;
+                if (std::fabs(prod) < 1e-10) return 0;
+              }
+  }
+  // upload in compute precision
+  if (ctx->fp32) {
+    std::vector<float> wf(nw);
+    for (size_t i = 0; i < nw; i++) wf[i] = (float)w[i];
+    float* d;
+    TRY(alloc(ctx, &d, nw));
+    NAT_HIP(hipMemcpy(d, wf.data(), nw * sizeof(float), hipMemcpyHostToDevice));
+    ctx->ax_w = d;
+  } else {
+    double* d;
+    TRY(alloc(ctx, &d, nw));
+    NAT_HIP(hipMemcpy(d, w.data(), nw * sizeof(double), hipMemcpyHostToDevice));
+    ctx->ax_w = d;
+  }
+  TRY(alloc(ctx, &ctx->ax_off, no));
+  NAT_HIP(hipMemcpy(ctx->ax_off, off.data(), no * sizeof(int), hipMemcpyHostToDevice));
+  *ok = true;
+  return 0;
+}
+
+bool structured_instantiated(int D, int Q, int p) {
+  const bool pq = (p == 1 || p == 2 || p == 3 || p == 4 || p == 6);
+  if (D == 3) return (Q == 19 || Q == 27) && pq;
+  return Q == 9 && pq;
+}
+
+int assemble_general(nat_ctx* ctx) {
+  const int Qm1 = ctx->Q - 1;
+  std::vector<double> disp(3 * Qm1, 0.0);
+  for (int a = 1; a < ctx->Q; a++)
+    for (int d = 0; d < 3; d++) disp[3 * (a - 1) + d] = ctx->st.dir[a][d] * (-ctx->dt);
+  double* ddisp;
+  TRY(alloc(ctx, &ddisp, disp.size()));
+  NAT_HIP(hipMemcpyAsync(ddisp, disp.data(), disp.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  const int64_t nrec = ctx->n_dofs * Qm1;
+  if (!ctx->dep_cell) TRY(alloc(ctx, &ctx->dep_cell, nrec));
+  if (!ctx->dep_xi64) TRY(alloc(ctx, &ctx->dep_xi64, nrec * ctx->D));
+  TRY(nat_launch_assemble_general(ctx, ddisp, ctx->dep_cell, ctx->dep_xi64));
+  if (ctx->fp32) {
+    if (!ctx->dep_xi32) TRY(alloc(ctx, &ctx->dep_xi32, nrec * ctx->D));
+    TRY(nat_launch_xi_to_f32(ctx, ctx->dep_xi64, ctx->dep_xi32, nrec * ctx->D));
+  }
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  hipFree(ddisp);
+  return check_status(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nat_last_error(const nat_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+int64_t nat_n_dofs(const nat_ctx* ctx) { return ctx->n_dofs; }
+int nat_n_stored_nodes(const nat_ctx* ctx, int64_t* n) {
+  *n = ctx->n_store;
+  return 0;
+}
+int nat_Q(const nat_ctx* ctx) { return ctx->Q; }
+double nat_dt(const nat_ctx* ctx) { return ctx->dt; }
+double nat_tau(const nat_ctx* ctx) { return ctx->tau; }
+double nat_time(const nat_ctx* ctx) { return ctx->time; }
+int nat_backend(const nat_ctx* ctx) { return ctx->backend; }
+
+void nat_destroy(nat_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  void* bufs[] = {ctx->f[0],     ctx->f[1],        ctx->g[0],     ctx->g[1],       ctx->rho,       ctx->u,
+                  ctx->T,        ctx->mask,        ctx->d_vert,   ctx->d_gll,      ctx->d_coords,  ctx->d_owner,
+                  ctx->d_status, ctx->d_sums,      ctx->dep_cell, ctx->dep_xi64,   ctx->dep_xi32,  ctx->ax_w,
+                  ctx->ax_off,   ctx->sedg_df,     ctx->sedg_geo};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int nat_create(const nat_desc* desc, nat_ctx** out) {
+  *out = nullptr;
+  nat_ctx* ctx = new (std::nothrow) nat_ctx;
+  if (!ctx) return NAT_ERR_NOMEM;
+  *out = ctx;
+  if (!desc) return fail(ctx, NAT_ERR_ARG, "null descriptor");
+  ctx->desc = *desc;
+  nat_desc& d = ctx->desc;
+  if (d.dim != 2 && d.dim != 3) return fail(ctx, NAT_ERR_ARG, "dim must be 2 or 3");
+  if (!make_stencil(d.stencil, d.scaling, ctx->st)) return fail(ctx, NAT_ERR_ARG, "unknown stencil");
+  if (ctx->st.D != d.dim) return fail(ctx, NAT_ERR_ARG, "stencil dimension does not match dim");
+  if (d.order < 1 || d.order > MAXP) return fail(ctx, NAT_ERR_ARG, "order must be in [1, 8]");
+  if (!(d.scaling > 0) || !(d.viscosity > 0) || !(d.cfl > 0)) return fail(ctx, NAT_ERR_ARG, "scaling, viscosity and cfl must be > 0");
+  if (!d.vertices) return fail(ctx, NAT_ERR_ARG, "vertices required");
+  if (d.precision != NAT_FP64 && d.precision != NAT_FP32_SHIFTED) return fail(ctx, NAT_ERR_ARG, "unknown precision");
+  if (d.scheme != NAT_SL && d.scheme != NAT_SEDG && d.scheme != NAT_SEDG_CENTRAL) return fail(ctx, NAT_ERR_ARG, "unknown scheme");
+  if (d.equilibrium != NAT_EQ_BGK && !d.compressible)
+    return fail(ctx, NAT_ERR_UNSUPPORTED, "incompressible collision supports the BGK equilibrium only");
+  if (d.compressible) return fail(ctx, NAT_ERR_UNSUPPORTED, "compressible populations are not enabled in this build");
+  ctx->D = d.dim;
+  ctx->Q = ctx->st.Q;
+  ctx->p = d.order;
+  ctx->fp32 = d.precision == NAT_FP32_SHIFTED;
+  for (int a = 0; a < 3; a++) {
+    if (a >= d.dim) {
+      d.ncells[a] = 1;
+      d.periodic[a] = 1;
+    }
+    if (d.ncells[a] < 1) return fail(ctx, NAT_ERR_ARG, "ncells must be >= 1");
+    if (!d.periodic[a]) return fail(ctx, NAT_ERR_UNSUPPORTED, "non-periodic (wall) boundaries are not supported");
+  }
+  const int nz = d.dim == 3 ? d.ncells[2] + 1 : 1;
+  const size_t nv = (size_t)(d.ncells[0] + 1) * (d.ncells[1] + 1) * nz;
+  ctx->verts.assign(3 * nv, 0.0);
+  for (size_t i = 0; i < nv; i++)
+    for (int a = 0; a < d.dim; a++) ctx->verts[3 * i + a] = d.vertices[i * d.dim + a];
+  d.vertices = nullptr;
+  if (d.scheme != NAT_SL) d.cartesian = 0;  // DG uses the multilinear map
+
+  // time step and relaxation time (CFDSolverUtilities.cpp:92-99, AuxiliaryCollisionFunctions.h:63-68)
+  ctx->h_min = min_vertex_distance(ctx);
+  ctx->dt = d.cfl * ctx->h_min / (ctx->st.max_speed * d.order * d.order);
+  ctx->tau = d.viscosity / (ctx->dt * ctx->st.cs2) + 0.5;
+
+  NAT_HIP(hipStreamCreate(&ctx->stream));
+  TRY(alloc(ctx, &ctx->d_status, 1));
+  TRY(alloc(ctx, &ctx->d_sums, 8));
+  NAT_HIP(hipMemset(ctx->d_status, 0, sizeof(int)));
+  TRY(alloc(ctx, &ctx->d_vert, ctx->verts.size()));
+  NAT_HIP(hipMemcpy(ctx->d_vert, ctx->verts.data(), ctx->verts.size() * sizeof(double), hipMemcpyHostToDevice));
+  double gll[MAXP + 1], gw[MAXP + 1];
+  gll_points(d.order, gll, gw);
+  TRY(alloc(ctx, &ctx->d_gll, MAXP + 1));
+  NAT_HIP(hipMemcpy(ctx->d_gll, gll, sizeof(gll), hipMemcpyHostToDevice));
+
+  if (d.scheme == NAT_SL) {
+    for (int a = 0; a < 3; a++) {
+      ctx->nfe[a] = a < d.dim ? d.ncells[a] * d.order + 1 : 1;
+      ctx->nu[a] = a < d.dim ? d.ncells[a] * d.order : 1;
+    }
+    ctx->n_dofs = (int64_t)ctx->nfe[0] * ctx->nfe[1] * ctx->nfe[2];
+    bool want_struct = d.backend != NAT_BACKEND_GENERAL && d.cartesian && is_uniform_cartesian(ctx) &&
+                       structured_instantiated(d.dim, ctx->Q, d.order);
+    if (d.backend == NAT_BACKEND_STRUCTURED && !want_struct)
+      return fail(ctx, NAT_ERR_UNSUPPORTED, "structured backend needs a uniform Cartesian periodic mesh and a supported (stencil, order)");
+    ctx->backend = want_struct ? NAT_BACKEND_STRUCTURED : NAT_BACKEND_GENERAL;
+    if (ctx->backend == NAT_BACKEND_STRUCTURED) {
+      bool ok = false;
+      TRY(setup_structured_tables(ctx, &ok));
+      if (!ok) {
+        if (d.backend == NAT_BACKEND_STRUCTURED)
+          return fail(ctx, NAT_ERR_UNSUPPORTED, "operator is not a separable one-cell-layer shift; use the general backend");
+        ctx->backend = NAT_BACKEND_GENERAL;
+      }
+    }
+    ctx->n_store = ctx->backend == NAT_BACKEND_STRUCTURED ? (int64_t)ctx->nu[0] * ctx->nu[1] * ctx->nu[2] : ctx->n_dofs;
+    std::vector<double> xyz;
+    std::vector<int> owner;
+    feq_points(ctx, xyz, owner);
+    TRY(alloc(ctx, &ctx->d_coords, xyz.size()));
+    NAT_HIP(hipMemcpy(ctx->d_coords, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice));
+    TRY(alloc(ctx, &ctx->d_owner, owner.size()));
+    NAT_HIP(hipMemcpy(ctx->d_owner, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice));
+  } else {
+    const int np = d.order + 1;
+    const int64_t ncell = (int64_t)d.ncells[0] * d.ncells[1] * (d.dim == 3 ? d.ncells[2] : 1);
+    ctx->n_dofs = ncell * (d.dim == 3 ? np * np * np : np * np);
+    ctx->n_store = ctx->n_dofs;
+    ctx->backend = NAT_BACKEND_GENERAL;
+  }
+  ctx->stride = (ctx->n_store + 127) / 128 * 128;
+  const size_t es = ctx->fp32 ? sizeof(float) : sizeof(double);
+  for (int b = 0; b < 2; b++) {
+    hipError_t e = hipMalloc(&ctx->f[b], es * ctx->stride * ctx->Q);
+    if (e != hipSuccess) return fail(ctx, NAT_ERR_NOMEM, "device allocation of populations failed");
+    NAT_HIP(hipMemset(ctx->f[b], 0, es * ctx->stride * ctx->Q));
+  }
+  {
+    hipError_t e = hipMalloc(&ctx->rho, es * ctx->n_store);
+    if (e == hipSuccess) e = hipMalloc(&ctx->u, es * ctx->n_store * ctx->D);
+    if (e != hipSuccess) return fail(ctx, NAT_ERR_NOMEM, "device allocation of macroscopic fields failed");
+    NAT_HIP(hipMemset(ctx->rho, 0, es * ctx->n_store));
+    NAT_HIP(hipMemset(ctx->u, 0, es * ctx->n_store * ctx->D));
+  }
+  if (d.scheme == NAT_SL) {
+    if (ctx->backend == NAT_BACKEND_GENERAL) TRY(assemble_general(ctx));
+  } else {
+    TRY(nat_sedg_setup(ctx));
+  }
+  ctx->assembled = true;
+  return 0;
+}
+
+int nat_node_coords(nat_ctx* ctx, double* xyz) {
+  std::vector<double> x;
+  if (ctx->desc.scheme == NAT_SL) {
+    std::vector<int> owner;
+    feq_points(ctx, x, owner);
+  } else {
+    dg_points(ctx, x);
+  }
+  std::memcpy(xyz, x.data(), x.size() * sizeof(double));
+  return 0;
+}
+
+int nat_set_dt(nat_ctx* ctx, double dt) {
+  if (dt > 0) {
+    ctx->dt = dt;
+    ctx->tau = ctx->desc.viscosity / (ctx->dt * ctx->st.cs2) + 0.5;
+  }
+  if (ctx->desc.scheme != NAT_SL) return nat_sedg_setup(ctx);
+  if (ctx->backend == NAT_BACKEND_STRUCTURED) {
+    if (ctx->ax_w) hipFree(ctx->ax_w);
+    if (ctx->ax_off) hipFree(ctx->ax_off);
+    ctx->ax_w = nullptr;
+    ctx->ax_off = nullptr;
+    bool ok = false;
+    TRY(setup_structured_tables(ctx, &ok));
+    if (!ok) return fail(ctx, NAT_ERR_UNSUPPORTED, "time step gives a non-separable operator for the structured backend");
+    return 0;
+  }
+  return assemble_general(ctx);
+}
+
+static int upload_pop(nat_ctx* ctx, void* dst, const double* f) {
+  const int Q = ctx->Q;
+  const int64_t n = ctx->n_dofs;
+  if (ctx->fp32) {
+    const double rho0 = ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0;
+    std::vector<float> h((size_t)ctx->stride * Q, 0.0f);
+    for (int q = 0; q < Q; q++)
+      for (int64_t i = 0; i < n; i++) h[(size_t)q * ctx->stride + store_index(ctx, i)] = (float)(f[q * n + i] - ctx->st.w[q] * rho0);
+    NAT_HIP(hipMemcpy(dst, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> h((size_t)ctx->stride * Q, 0.0);
+    for (int q = 0; q < Q; q++)
+      for (int64_t i = 0; i < n; i++) h[(size_t)q * ctx->stride + store_index(ctx, i)] = f[q * n + i];
+    NAT_HIP(hipMemcpy(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+static int download_pop(nat_ctx* ctx, const void* src, double* f) {
+  const int Q = ctx->Q;
+  const int64_t n = ctx->n_dofs;
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->fp32) {
+    const double rho0 = ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0;
+    std::vector<float> h((size_t)ctx->stride * Q);
+    NAT_HIP(hipMemcpy(h.data(), src, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int q = 0; q < Q; q++)
+      for (int64_t i = 0; i < n; i++) f[q * n + i] = (double)h[(size_t)q * ctx->stride + store_index(ctx, i)] + ctx->st.w[q] * rho0;
+  } else {
+    std::vector<double> h((size_t)ctx->stride * Q);
+    NAT_HIP(hipMemcpy(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int q = 0; q < Q; q++)
+      for (int64_t i = 0; i < n; i++) f[q * n + i] = h[(size_t)q * ctx->stride + store_index(ctx, i)];
+  }
+  return 0;
+}
+
+int nat_upload_f(nat_ctx* ctx, const double* f) {
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  return upload_pop(ctx, ctx->f[ctx->cur], f);
+}
+int nat_download_f(nat_ctx* ctx, double* f) { return download_pop(ctx, ctx->f[ctx->cur], f); }
+int nat_upload_g(nat_ctx* ctx, const double*) { return fail(ctx, NAT_ERR_UNSUPPORTED, "no g populations in this context"); }
+int nat_download_g(nat_ctx* ctx, double*) { return fail(ctx, NAT_ERR_UNSUPPORTED, "no g populations in this context"); }
+
+int nat_init_equilibrium(nat_ctx* ctx, const double* rho, const double* u, const double* T) {
+  (void)T;
+  // BGKStandard::getEquilibriumDistribution with the scaled stencil (BGKStandard.cpp:23-41)
+  const int Q = ctx->Q, D = ctx->D;
+  const int64_t n = ctx->n_dofs;
+  std::vector<double> f((size_t)Q * n);
+  const double cs2 = ctx->st.cs2;
+  for (int64_t i = 0; i < n; i++) {
+    double uu = 0;
+    for (int d = 0; d < D; d++) uu += u[d * n + i] * u[d * n + i];
+    for (int q = 0; q < Q; q++) {
+      const double pre = ctx->st.w[q] * rho[i];
+      const double usq = -(uu) / (2 * cs2);
+      if (q == 0) {
+        f[(size_t)q * n + i] = pre * (1 + usq);
+        continue;
+      }
+      double ue = 0;
+      for (int d = 0; d < D; d++) ue += u[d * n + i] * ctx->st.dir[q][d];
+      const double m = ue / cs2;
+      f[(size_t)q * n + i] = pre * (1 + m * (1 + 0.5 * m) + usq);
+    }
+  }
+  ctx->time = 0;
+  ctx->collided = false;
+  return nat_upload_f(ctx, f.data());
+}
+
+int nat_collide(nat_ctx* ctx) {
+  TRY(nat_launch_collide(ctx));
+  ctx->collided = true;
+  return check_status(ctx);
+}
+
+int nat_step(nat_ctx* ctx, int nsteps) {
+  if (!ctx->assembled) return fail(ctx, NAT_ERR_STATE, "operator not assembled");
+  for (int s = 0; s < nsteps; s++) {
+    const int wm = (s == nsteps - 1) ? 1 : 0;
+    if (ctx->desc.scheme == NAT_SL) {
+      if (ctx->backend == NAT_BACKEND_STRUCTURED)
+        TRY(nat_launch_sl_struct(ctx, wm));
+      else
+        TRY(nat_launch_sl_general(ctx, wm));
+    } else {
+      TRY(nat_sedg_step(ctx, 1, wm));
+    }
+    ctx->time += ctx->dt;
+  }
+  return check_status(ctx);
+}
+
+int nat_stream(nat_ctx* ctx) {
+  if (ctx->desc.scheme == NAT_SL) return fail(ctx, NAT_ERR_UNSUPPORTED, "SL streaming is fused with collision: use nat_step");
+  TRY(nat_sedg_step(ctx, 0, 0));
+  ctx->time += ctx->dt;
+  return check_status(ctx);
+}
+
+int nat_get_macro(nat_ctx* ctx, double* rho, double* u, double* T) {
+  (void)T;
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  const int64_t n = ctx->n_dofs, ns = ctx->n_store;
+  const int D = ctx->D;
+  if (ctx->fp32) {
+    std::vector<float> r(ns), v((size_t)ns * D);
+    NAT_HIP(hipMemcpy(r.data(), ctx->rho, ns * sizeof(float), hipMemcpyDeviceToHost));
+    NAT_HIP(hipMemcpy(v.data(), ctx->u, ns * D * sizeof(float), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) {
+      const int64_t s = store_index(ctx, i);
+      rho[i] = r[s];
+      for (int d = 0; d < D; d++) u[d * n + i] = v[(size_t)d * ns + s];
+    }
+  } else {
+    std::vector<double> r(ns), v((size_t)ns * D);
+    NAT_HIP(hipMemcpy(r.data(), ctx->rho, ns * sizeof(double), hipMemcpyDeviceToHost));
+    NAT_HIP(hipMemcpy(v.data(), ctx->u, ns * D * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) {
+      const int64_t s = store_index(ctx, i);
+      rho[i] = r[s];
+      for (int d = 0; d < D; d++) u[d * n + i] = v[(size_t)d * ns + s];
+    }
+  }
+  return 0;
+}
+
+int nat_sums(nat_ctx* ctx, double* mass, double* momentum, double* ekin) {
+  const int layout = ctx->desc.scheme != NAT_SL ? 2 : (ctx->backend == NAT_BACKEND_STRUCTURED ? 1 : 0);
+  TRY(nat_launch_sums(ctx, layout));
+  double h[5];
+  NAT_HIP(hipMemcpyAsync(h, ctx->d_sums, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  if (mass) *mass = h[0];
+  if (momentum)
+    for (int d = 0; d < ctx->D; d++) momentum[d] = h[1 + d];
+  if (ekin) *ekin = 0.5 * h[4];
+  return 0;
+}
+
+int nat_sync(nat_ctx* ctx) {
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+"""Problem descriptions on the hot path: lattice meshes and Taylor-Green initial fields.
+
+Host-side mirror of the reference benchmarks used by the solver configurations:
+  benchmarks/TaylorGreenVortex2D.cpp:36-97  (fields, hyper_cube [0, L]^2, all periodic)
+  benchmarks/TaylorGreenVortex3D.cpp:38-119 (fields, [0, 2 pi]^3, all periodic)
+and of the solver parameter conventions
+  utilities/CFDSolverUtilities.cpp:92-99     dt = CFL h_min / (|c|max p^2)
+  collision_advanced/AuxiliaryCollisionFunctions.h:63-68  tau = nu / (dt cs2) + 1/2
+"""
+import math
+
+import numpy as np
+
+
+def box_vertices(dim, ncells, lo, hi):
+    """Vertices of a lattice box mesh, lexicographic (x fastest), shape (nv, dim)."""
+    axes = [lo[a] + (hi[a] - lo[a]) * np.arange(ncells[a] + 1, dtype=np.float64) / ncells[a] for a in range(dim)]
+    grids = np.meshgrid(*axes, indexing="ij")
+    # lexicographic with x fastest: transpose so the last index varies slowest
+    cols = [g.transpose(tuple(reversed(range(dim)))).reshape(-1) for g in grids]
+    return np.stack(cols, axis=1)
+
+
+def perturbed_vertices(dim, ncells, lo, hi, amplitude=0.1, seed=1234):
+    """Box vertices with every interior vertex moved uniformly by +-amplitude*h per component.
+
+    Generator: numpy.random.default_rng(seed).uniform(-1, 1) drawn in vertex order, one value per
+    (vertex, component), for all vertices; boundary vertices discard their draws so periodic faces match.
+    """
+    v = box_vertices(dim, ncells, lo, hi)
+    h = np.array([(hi[a] - lo[a]) / ncells[a] for a in range(dim)])
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-1.0, 1.0, size=v.shape) * amplitude * h
+    idx = np.stack(np.unravel_index(np.arange(v.shape[0]), tuple(ncells[a] + 1 for a in reversed(range(dim)))), 1)
+    idx = idx[:, ::-1]  # (i, j[, k]) with x fastest
+    interior = np.ones(v.shape[0], dtype=bool)
+    for a in range(dim):
+        interior &= (idx[:, a] > 0) & (idx[:, a] < ncells[a])
+    v[interior] += d[interior]
+    return v
+
+
+def lbm_parameters(U=1.0, Ma=0.1):
+    """Reference driver convention: cs = U/Ma, stencil scaling s = sqrt(3) cs."""
+    cs = U / Ma
+    return cs, math.sqrt(3.0) * cs
+
+
+def tgv2d_fields(x, viscosity, cs, L=2 * math.pi, t=0.0, rho_analytic=True):
+    """TaylorGreenVortex2D::AnalyticVelocity / AnalyticDensity (TaylorGreenVortex2D.cpp:36-58)."""
+    X, Y = x[:, 0], x[:, 1]
+    k = (2 * math.pi)
+    dec = math.exp(-2 * viscosity * t * k / L)
+    u = np.empty((2, x.shape[0]))
+    u[0] = np.sin(X * k / L) * np.cos(Y * k / L) * dec
+    u[1] = -np.cos(X * k / L) * np.sin(Y * k / L) * dec
+    if rho_analytic:
+        p = 1.0 / 4.0 * (np.cos(2 * (X * k / L)) + np.cos(2 * Y * k / L)) * math.exp(-4 * viscosity * t * k / L)
+        rho = 1.0 + p / (cs * cs)
+    else:
+        rho = np.ones(x.shape[0])
+    return rho, u
+
+
+def tgv3d_fields(x, cs, rho_analytic=True):
+    """TaylorGreenVortex3D::InitialVelocity / InitialDensity (TaylorGreenVortex3D.cpp:38-59)."""
+    X, Y, Z = x[:, 0], x[:, 1], x[:, 2]
+    u = np.empty((3, x.shape[0]))
+    u[0] = np.sin(X) * np.cos(Y) * np.cos(Z)
+    u[1] = -np.cos(X) * np.sin(Y) * np.cos(Z)
+    u[2] = 0.0
+    if rho_analytic:
+        p = 1.0 / 16.0 * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2)
+        rho = 1.0 + p / (cs * cs)
+    else:
+        rho = np.ones(x.shape[0])
+    return rho, u
+
+
+def riemann_quadrant_fields(x, states, lo=(0.0, 0.0), hi=(1.0, 1.0)):
+    """Four-quadrant piecewise-constant (rho, u, v, p) -> (rho, u, T = p/rho)."""
+    xm = 0.5 * (lo[0] + hi[0])
+    ym = 0.5 * (lo[1] + hi[1])
+    q = (x[:, 0] >= xm).astype(int) + 2 * (x[:, 1] >= ym).astype(int)
+    st = np.asarray(states, dtype=np.float64)  # 4 x (rho, u, v, p): quadrants SW, SE, NW, NE
+    rho = st[q, 0]
+    u = np.stack([st[q, 1], st[q, 2]])
+    T = st[q, 3] / st[q, 0]
+    return rho, u, T
+
+
+def riemann_states(seed=7):
+    """Seeded quadrant states: rho in [0.5,1.5], p in [0.3,1.5], |u| <= 0.7."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(4):
+        rho = rng.uniform(0.5, 1.5)
+        p = rng.uniform(0.3, 1.5)
+        ang = rng.uniform(0, 2 * math.pi)
+        mag = rng.uniform(0, 0.7)
+        out.append((rho, mag * math.cos(ang), mag * math.sin(ang), p))
+    return out
+
+
+def divergence_free_field(x, modes=8, amplitude=0.05, seed=1234, L=2 * math.pi):
+    """Seeded random divergence-free Fourier velocity field (3D): sum of curl-free-projected modes."""
+    rng = np.random.default_rng(seed)
+    u = np.zeros((3, x.shape[0]))
+    kk = 2 * math.pi / L
+    for _ in range(modes * 3):
+        k = rng.integers(-modes, modes + 1, size=3).astype(np.float64)
+        if not k.any():
+            continue
+        a = rng.normal(size=3)
+        a -= k * (a @ k) / (k @ k)  # project out the longitudinal part
+        ph = rng.uniform(0, 2 * math.pi)
+        s = np.sin(kk * (x @ k) + ph)
+        u += np.outer(a, s)
+    nrm = np.sqrt((u ** 2).sum(0).mean())
+    return u * (amplitude / max(nrm, 1e-300))

@@ -1,0 +1,46 @@
+// Oracle collision operators. TEST INFRASTRUCTURE ONLY.
+//
+// Reference (src/library/natrium/):
+//   collision_advanced/CollisionOperator.h:26-111   incompressible collideAll
+//   collision_advanced/CollisionOperator.h:113-224  compressible collideAll (f, g)
+//   collision_advanced/CollisionSchemes.h:28-41     BGKCollision::relax
+//   collision_advanced/CollisionSchemes.h:43-118    BGKCollision::relaxWithG
+//   collision_advanced/Equilibria.h:34-83           BGKEquilibrium (D2Q9 specialised + generic)
+//   collision_advanced/Equilibria.h:120-267         QuarticEquilibrium::polynomial
+//   collision_advanced/AuxiliaryCollisionFunctions.h:46-68,106-202,232-307,421-431,475-481,520-566
+//   collision/BGKStandard.cpp:23-41                 legacy equilibrium used for initial f
+//   solver/CompressibleCFDSolver.h:790-848,850-1004 compressible initial f, g
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "orc_core.hpp"
+
+namespace orc {
+
+struct CollisionParams {
+  double scaling = 1, viscosity = 0, dt = 0, gamma = 1.4;
+  bool prandtl_set = false, sutherland_set = false;
+  double prandtl = 1.0;
+};
+
+// f: Q arrays of n; rho: n; u: D arrays of n (written as u*scaling).
+// Returns 0, or 1 if a density fell below 1e-10 (the reference throws).
+int collide_bgk(const Stencil& st, const CollisionParams& cp, int64_t n, double* const* f, double* rho,
+                double* const* u);
+
+// Compressible BGK with quartic equilibrium on f and g; writes rho, u*scaling, T, mask.
+int collide_compressible(const Stencil& st, const CollisionParams& cp, int64_t n, double* const* f, double* const* g,
+                         double* rho, double* const* u, double* T, double* mask);
+
+// f = legacy BGKStandard equilibrium (scaled stencil) of (rho, u physical).
+void init_equilibrium(const Stencil& st, int64_t n, const double* rho, const double* const* u, double* const* f);
+
+// compressible initial f (quartic, loop form) and g = f T (2 C_v - D).
+void init_compressible(const Stencil& st, const CollisionParams& cp, int64_t n, const double* rho,
+                       const double* const* u, const double* T, double* const* f, double* const* g);
+
+// quartic equilibrium (polynomial form) for one node, e unscaled, cs2 unscaled
+void quartic_feq(const Stencil& st, double rho, const double u[3], double T, double* feq);
+
+}  // namespace orc

@@ -1,0 +1,156 @@
+"""GPU parity: fused SL stream + BGK collide through the C ABI vs the FP64 oracle.
+
+Tolerances: FP64 populations within 1e-12 relative (max |f_gpu - f_oracle| / max |f_oracle|)
+at every checked step; FP32-shifted velocities within the per-config bound stated in each test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import natrium_amd as nat
+from natrium_amd.problems import box_vertices, lbm_parameters, perturbed_vertices, tgv2d_fields, tgv3d_fields
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _pair(stencil, p, N, verts, cartesian, scaling, nu, precision="fp64", backend="auto", cfl=0.4):
+    g = nat.Solver(stencil, p, N, verts, cartesian=cartesian, scaling=scaling, viscosity=nu, cfl=cfl,
+                   precision=precision, backend=backend)
+    o = Oracle(stencil, p, N, verts, cartesian=cartesian, scaling=scaling, viscosity=nu, cfl=cfl)
+    o.assemble()
+    return g, o
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _tgv2d(backend, nsteps, check_every=1):
+    cs, s = lbm_parameters(1.0, 0.05)
+    L = 2 * math.pi
+    nu = L / 10.0
+    v = box_vertices(2, [8, 8], [0, 0], [L, L])
+    g, o = _pair("D2Q9", 4, [8, 8], v, True, s, nu, backend=backend)
+    assert g.n == o.n == 33 * 33
+    assert abs(g.dt - o.dt) < 1e-18 and abs(g.tau - o.tau) < 1e-12
+    x = g.coords()
+    np.testing.assert_array_equal(x, o.coords())
+    rho, u = tgv2d_fields(x, nu, cs, L)
+    g.init_equilibrium(rho, u)
+    o.init(rho, u)
+    assert _rel(g.download_f(), o.get_f()) == 0.0
+    g.collide()
+    o.collide()
+    assert _rel(g.download_f(), o.get_f()) < FP64_TOL
+    worst = 0.0
+    done = 0
+    while done < nsteps:
+        k = min(check_every, nsteps - done)
+        g.step(k)
+        o.steps(k)
+        done += k
+        e = _rel(g.download_f(), o.get_f())
+        worst = max(worst, e)
+        assert e < FP64_TOL, f"step {done}: rel diff {e}"
+    return g, o, worst
+
+
+@pytest.mark.parametrize("backend", ["structured", "general"])
+def test_tgv2d_d2q9_p4_fp64_per_step(backend):
+    g, o, worst = _tgv2d(backend, 20, check_every=1)
+    assert g.backend_name == backend
+
+
+@pytest.mark.parametrize("backend", ["structured", "general"])
+def test_tgv2d_d2q9_p4_fp64_100_steps_and_sums(backend):
+    g, o, worst = _tgv2d(backend, 100, check_every=10)
+    m_g, mom_g, e_g = g.sums()
+    m_o, e_o = o.sums()
+    assert abs(m_g - m_o) / m_o < 1e-12
+    assert abs(e_g - e_o) / e_o < 1e-10
+    rg, ug, _ = g.macro()
+    ro, uo, _, _ = o.macro()
+    assert _rel(rg, ro) < FP64_TOL
+    assert _rel(ug, uo) < 1e-10
+
+
+@pytest.mark.parametrize("stencil,p,N", [("D3Q27", 4, 4), ("D3Q19", 4, 4), ("D3Q27", 2, 4), ("D3Q19", 3, 4)])
+@pytest.mark.parametrize("backend", ["structured", "general"])
+def test_tgv3d_fp64_parity(stencil, p, N, backend):
+    cs, s = lbm_parameters(1.0, 0.1)
+    L = 2 * math.pi
+    v = box_vertices(3, [N] * 3, [0] * 3, [L] * 3)
+    g, o = _pair(stencil, p, [N] * 3, v, True, s, 1.0 / 400, backend=backend)
+    assert g.backend_name == backend
+    x = g.coords()
+    rho, u = tgv3d_fields(x, cs)
+    g.init_equilibrium(rho, u)
+    o.init(rho, u)
+    g.collide()
+    o.collide()
+    for _ in range(3):
+        g.step(4)
+        o.steps(4)
+        e = _rel(g.download_f(), o.get_f())
+        assert e < FP64_TOL, e
+
+
+@pytest.mark.parametrize("stencil,p", [("D3Q19", 3), ("D3Q27", 2)])
+def test_perturbed_mesh_general_fp64_parity(stencil, p):
+    cs, s = lbm_parameters(1.0, 0.05)
+    L = 2 * math.pi
+    N = [3, 3, 3]
+    v = perturbed_vertices(3, N, [0] * 3, [L] * 3, amplitude=0.1, seed=1234)
+    g, o = _pair(stencil, p, N, v, False, s, 2 * math.pi / 100, backend="auto")
+    assert g.backend_name == "general"
+    x = g.coords()
+    np.testing.assert_allclose(x, o.coords(), rtol=0, atol=1e-14)
+    rho, u = tgv3d_fields(x, cs)
+    g.init_equilibrium(rho, u)
+    o.init(rho, u)
+    g.collide()
+    o.collide()
+    for _ in range(2):
+        g.step(5)
+        o.steps(5)
+        e = _rel(g.download_f(), o.get_f())
+        assert e < FP64_TOL, e
+
+
+def test_fp32_shifted_tgv2d_velocity_error():
+    """FP32-shifted populations vs the FP64 oracle on the 2D TGV config: 100 steps.
+
+    Tolerance 1e-6 on the relative velocity L2 error (the FP32 bulk budget)."""
+    cs, s = lbm_parameters(1.0, 0.05)
+    L = 2 * math.pi
+    nu = L / 10.0
+    v = box_vertices(2, [8, 8], [0, 0], [L, L])
+    for backend in ("structured", "general"):
+        g, o = _pair("D2Q9", 4, [8, 8], v, True, s, nu, precision="fp32", backend=backend)
+        x = g.coords()
+        rho, u = tgv2d_fields(x, nu, cs, L)
+        g.init_equilibrium(rho, u)
+        o.init(rho, u)
+        g.collide()
+        o.collide()
+        g.step(100)
+        o.steps(100)
+        _, ug, _ = g.macro()
+        _, uo, _, _ = o.macro()
+        err = np.sqrt(((ug - uo) ** 2).sum() / (uo ** 2).sum())
+        assert err < 1e-6, (backend, err)
+
+
+def test_density_error_is_reported():
+    cs, s = lbm_parameters(1.0, 0.05)
+    L = 2 * math.pi
+    v = box_vertices(2, [2, 2], [0, 0], [L, L])
+    g = nat.Solver("D2Q9", 2, [2, 2], v, scaling=s, viscosity=1.0)
+    f = np.zeros((9, g.n))
+    g.upload_f(f)
+    with pytest.raises(nat.DensityZeroError):
+        g.collide()
