@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Headline benchmark: fused semi-Lagrangian stream + BGK collide, FP32-shifted vs FP64.
+
+Workload (one MI355X): 3D Taylor-Green vortex, [0, 2 pi]^3, 16^3 affine hexes, FE_Q order p = 4,
+D3Q27, Re = 400 (nu = 1/400), Ma = 0.1, CFL 0.4. One step = one fused stream + collide over all
+nodes, populations resident in HBM. Reports GLUPS (lattice-node updates per second) of the
+FP32-shifted path, the FP64 path beside it, the FP32 velocity L2 error vs FP64 after 1000 steps,
+the HBM roofline of the fused kernel and the CPU oracle timed on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 100]
+"""
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GLUPS FP32-shifted vs FP64 at L2 err≤1e-6 (TGV 3D); p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def tgv3d_problem(N=16, p=4, Ma=0.1, Re=400.0):
+    from natrium_amd.problems import box_vertices, lbm_parameters
+
+    cs, s = lbm_parameters(1.0, Ma)
+    L = 2 * math.pi
+    v = box_vertices(3, [N] * 3, [0.0] * 3, [L] * 3)
+    return dict(stencil="D3Q27", order=p, ncells=[N] * 3, vertices=v, scaling=s, viscosity=1.0 / Re, cfl=0.4,
+                cs=cs)
+
+
+def make_solver(nat, prob, precision):
+    return nat.Solver(prob["stencil"], prob["order"], prob["ncells"], prob["vertices"], cartesian=True,
+                      scaling=prob["scaling"], viscosity=prob["viscosity"], cfl=prob["cfl"], precision=precision)
+
+
+def init_tgv(sol, prob):
+    from natrium_amd.problems import tgv3d_fields
+
+    x = sol.coords()
+    rho, u = tgv3d_fields(x, prob["cs"])
+    sol.init_equilibrium(rho, u)
+    sol.collide()
+
+
+def timed_glups(sol, steps, warmup):
+    sol.step(warmup)
+    sol.sync()
+    t0 = time.perf_counter()
+    ms_dev = sol.step_timed(steps)
+    sol.sync()
+    wall = time.perf_counter() - t0
+    return wall, ms_dev
+
+
+def read_traffic(workload):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_" + workload + ".md")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(prob, seconds):
+    """The oracle (FP64 CPU restatement) on the same workload: assembly untimed, then whole
+    stream + collide steps until about `seconds` of wall time."""
+    from natrium_amd.problems import tgv3d_fields
+    from oracle import oracle as O
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    O.set_threads(threads)
+    o = O.Oracle(prob["stencil"], prob["order"], prob["ncells"], prob["vertices"], cartesian=True,
+                 scaling=prob["scaling"], viscosity=prob["viscosity"], cfl=prob["cfl"])
+    o.assemble()
+    x = o.coords()
+    rho, u = tgv3d_fields(x, prob["cs"])
+    o.init(rho, u)
+    o.collide()
+    o.steps(1)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.steps(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": o.n * n / el / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
+            "sample": f"{n} FP64 steps (CSR SpMV stream + BGK collide) of the same 16^3 p=4 D3Q27 TGV, "
+                      f"{o.n} FE_Q DoFs, {el:.1f} s wall, OpenMP {threads} threads; assembly untimed"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--error-steps", type=int, default=1000)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the hot path runs on a single MI355X; "
+              "multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    import natrium_amd as nat
+
+    prob = tgv3d_problem()
+    Q = 27
+    workload = "tgv3d_16c_p4_d3q27"
+
+    # FP32-shifted: the headline
+    s32 = make_solver(nat, prob, "fp32")
+    assert s32.backend_name == "structured"
+    n_unique = s32.n_stored
+    n_dofs = s32.n
+    init_tgv(s32, prob)
+    wall32, ms32 = timed_glups(s32, args.steps, args.warmup)
+    glups32 = n_unique * args.steps / wall32 / 1e9
+
+    # FP64 beside it
+    s64 = make_solver(nat, prob, "fp64")
+    init_tgv(s64, prob)
+    wall64, ms64 = timed_glups(s64, args.steps, args.warmup)
+    glups64 = n_unique * args.steps / wall64 / 1e9
+
+    # accuracy: FP32-shifted vs FP64 after error-steps steps from the same initial state
+    import numpy as np
+
+    a32 = make_solver(nat, prob, "fp32")
+    a64 = make_solver(nat, prob, "fp64")
+    init_tgv(a32, prob)
+    init_tgv(a64, prob)
+    _, _, e0 = a64.sums()
+    a32.step(args.error_steps)
+    a64.step(args.error_steps)
+    _, u32, _ = a32.macro()
+    _, u64, _ = a64.macro()
+    l2 = float(np.sqrt(((u32 - u64) ** 2).sum() / (u64 ** 2).sum()))
+    _, _, e1 = a64.sums()
+    t_end = a64.time
+
+    # roofline of the fused kernel: every population read once and written once per launch
+    bytes_per_launch = n_unique * 2 * Q * 4
+    avg_launch_s = ms32 / 1e3 / args.steps
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(workload),
+                "kernel": "k_sl_struct<3,27,4,float>", "bytes_per_launch": bytes_per_launch,
+                "avg_launch_us": round(avg_launch_s * 1e6, 3)}
+
+    cpu = None if args.no_cpu else cpu_baseline(prob, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(glups32, 4),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall32 * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": workload, "mesh": "[0,2pi]^3 16^3 affine hexes", "order": 4, "stencil": "D3Q27",
+                   "Re": 400, "Ma": 0.1, "cfl": 0.4, "nodes_updated_per_step": n_unique,
+                   "reference_fe_q_dofs": n_dofs, "parallelism": "single GPU"},
+        "fp64_glups": round(glups64, 4),
+        "fp32_over_fp64": round(glups32 / glups64, 3),
+        "glups_reference_dof_convention": round(n_dofs * args.steps / wall32 / 1e9, 4),
+        "fp64_ms_per_step": round(wall64 * 1e3 / args.steps, 5),
+        "l2_err_u_fp32_vs_fp64": l2,
+        "l2_err_steps": args.error_steps,
+        "ekin_ratio_fp64": e1 / e0,
+        "ekin_ratio_stokes": math.exp(-6 * prob["viscosity"] * t_end),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "latency_p50_ms": None,
+        "latency_p99_ms": None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -112,6 +112,12 @@ int nat_stream(nat_ctx* ctx);
  * runs the initial collide() of run() once with nat_collide(). */
 int nat_step(nat_ctx* ctx, int nsteps);
 
+/* Measurement helper with no counterpart in the reference (its solver prints
+ * "million DoF updates per second" from clock(), CFDSolver.cpp:975-982): runs
+ * nat_step(ctx, nsteps) between two HIP events recorded on the context's stream
+ * and returns the elapsed device time in milliseconds. */
+int nat_step_timed(nat_ctx* ctx, int nsteps, double* elapsed_ms);
+
 /* macroscopic fields written by the last collision (FP64 host arrays; T may be NULL) */
 int nat_get_macro(nat_ctx* ctx, double* rho, double* u, double* T);
 

@@ -45,7 +45,7 @@ EXPORTS = [
     "nat_create", "nat_destroy", "nat_last_error", "nat_n_dofs", "nat_n_stored_nodes", "nat_Q", "nat_dt",
     "nat_tau", "nat_time", "nat_backend", "nat_node_coords", "nat_set_dt", "nat_init_equilibrium",
     "nat_upload_f", "nat_download_f", "nat_upload_g", "nat_download_g", "nat_collide", "nat_stream",
-    "nat_step", "nat_get_macro", "nat_sums", "nat_sync",
+    "nat_step", "nat_step_timed", "nat_get_macro", "nat_sums", "nat_sync",
 ]
 
 _lib = None
@@ -87,6 +87,8 @@ def lib():
         getattr(L, nm).argtypes = [vp, _dp]
     L.nat_step.restype = ctypes.c_int
     L.nat_step.argtypes = [vp, ctypes.c_int]
+    L.nat_step_timed.restype = ctypes.c_int
+    L.nat_step_timed.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.nat_get_macro.restype = ctypes.c_int
     L.nat_get_macro.argtypes = [vp, _dp, _dp, _dp]
     L.nat_sums.restype = ctypes.c_int
@@ -229,6 +231,12 @@ class Solver:
 
     def step(self, n=1):
         self._check(lib().nat_step(self._h, int(n)))
+
+    def step_timed(self, n):
+        """Run n fused steps; returns the device time in ms (HIP events on the library stream)."""
+        ms = ctypes.c_double()
+        self._check(lib().nat_step_timed(self._h, int(n), ctypes.byref(ms)))
+        return ms.value
 
     def macro(self):
         rho = np.zeros(self.n)

@@ -350,15 +350,35 @@ int nat_launch_assemble_axis(nat_ctx* ctx, const int* d_N, const double* d_vaxis
 }
 
 namespace nat {
-__global__ void k_xi_to_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = (float)src[i];
+// FP32 departure records: code = cell * (p+1)^D + anchor (the GLL node nearest
+// to the departure point, per axis, x fastest) and dxi = xi - x_anchor per axis.
+// The small offset keeps FP32 relative precision where the weights need it.
+__global__ void k_records_f32(const int* __restrict__ cell, const double* __restrict__ xi, int64_t nrec, int D,
+                              int p, GllTab g, int* __restrict__ code, float* __restrict__ dxi) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  const int c = cell[r];
+  int lin = 0, mult = 1;
+  for (int d = 0; d < D; d++) {
+    const double x = xi[r * D + d];
+    int best = 0;
+    for (int m = 1; m <= p; m++)
+      if (fabs(x - g.x[m]) < fabs(x - g.x[best])) best = m;
+    dxi[r * D + d] = (float)(x - g.x[best]);
+    lin += best * mult;
+    mult *= p + 1;
+  }
+  code[r] = c < 0 ? -1 : c * mult + lin;
 }
 }  // namespace nat
 
-int nat_launch_xi_to_f32(nat_ctx* ctx, const double* src, float* dst, int64_t n) {
+int nat_launch_records_f32(nat_ctx* ctx, int64_t nrec) {
+  nat::GllTab g;
+  double w[nat::MAXP + 1];
+  nat::gll_points(ctx->p, g.x, w);
   const int bs = 256;
-  hipLaunchKernelGGL(nat::k_xi_to_f32, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, ctx->stream, src, dst, n);
+  hipLaunchKernelGGL(nat::k_records_f32, dim3((unsigned)((nrec + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
+                     ctx->dep_cell, ctx->dep_xi64, nrec, ctx->D, ctx->p, g, ctx->dep_code32, ctx->dep_xi32);
   NAT_HIP(hipGetLastError());
   return 0;
 }

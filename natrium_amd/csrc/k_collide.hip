@@ -2,6 +2,7 @@
 // conservation sums (PhysicalProperties::mass / kineticEnergy,
 // PhysicalProperties.cpp:29-130 and 270-310).
 #include "nat_collide.hpp"
+#include "nat_geom.hpp"
 
 namespace nat {
 
@@ -29,34 +30,6 @@ struct SumArgs {
   double gll[MAXP + 1];
   double gllw[MAXP + 1];
 };
-
-__device__ __forceinline__ void cell_jac(const MeshDev& m, const int c[3], const double xi[3], double J[3][3]) {
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) J[a][b] = (a == b && a >= m.D) ? 1.0 : 0.0;
-  const int nv = m.D == 3 ? 8 : 4;
-  auto V = [&](int v) {
-    return m.vert + 3 * ((size_t)(c[0] + (v & 1)) +
-                         (size_t)(m.N[0] + 1) * ((size_t)(c[1] + ((v >> 1) & 1)) +
-                                                 (size_t)(m.N[1] + 1) * (m.D == 3 ? c[2] + ((v >> 2) & 1) : 0)));
-  };
-  if (m.cartesian) {
-    const double* a0 = V(0);
-    const double* b0 = V(nv - 1);
-    for (int d = 0; d < m.D; d++) J[d][d] = b0[d] - a0[d];
-    return;
-  }
-  for (int v = 0; v < nv; v++) {
-    const double* X = V(v);
-    for (int b = 0; b < m.D; b++) {
-      double dN = 1;
-      for (int a = 0; a < m.D; a++) {
-        const int bit = (v >> a) & 1;
-        dN *= (a == b) ? (bit ? 1.0 : -1.0) : (bit ? xi[a] : 1 - xi[a]);
-      }
-      for (int d = 0; d < m.D; d++) J[d][b] += dN * X[d];
-    }
-  }
-}
 
 // one thread per (cell, local node); block partial sums -> FP64 atomics
 template <typename T>
@@ -122,6 +95,7 @@ int nat_launch_collide(nat_ctx* ctx) {
   if (ctx->fp32) {
     CollideArgs<float> ca;
     ca.omega = (float)(1.0 / ctx->tau);
+    ca.omega64 = 1.0 / ctx->tau;
     ca.rho0 = (float)(ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0);
     ca.scaling = ctx->st.s;
     ca.rho = (float*)ctx->rho;
@@ -136,6 +110,7 @@ int nat_launch_collide(nat_ctx* ctx) {
   } else {
     CollideArgs<double> ca;
     ca.omega = 1.0 / ctx->tau;
+    ca.omega64 = 1.0 / ctx->tau;
     ca.rho0 = 0.0;
     ca.scaling = ctx->st.s;
     ca.rho = (double*)ctx->rho;

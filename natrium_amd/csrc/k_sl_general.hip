@@ -20,10 +20,12 @@ struct GeneralArgs {
   int64_t n;          // FE_Q DoFs
   int nfe[3];         // FE_Q lattice nodes per axis
   int N[3];           // cells per axis
-  const int* cell;    // [(alpha-1) * n + i]
-  const XT* xi;       // [((alpha-1) * n + i) * D + d]
+  const int* cell;    // FP64: [(alpha-1) * n + i] departure cell (-1: identity row)
+  const XT* xi;       // FP64: unit coordinates; FP32: offsets from the anchor node, D per record
+  const int* code;    // FP32: cell * (p+1)^D + anchor (-1: identity row)
   T gll[MAXP + 1];
   T inv_den[MAXP + 1];
+  T dgl[MAXP + 1][MAXP + 1];  // x_a - x_k
 };
 
 template <int P, typename T>
@@ -48,41 +50,96 @@ __global__ __launch_bounds__(256) void k_sl_general(GeneralArgs<T, XT> a, Collid
 #pragma unroll
   for (int al = 1; al < Q; al++) {
     const int64_t r = (int64_t)(al - 1) * a.n + i;
-    const int c = a.cell[r];
     const T* src = a.fin + (int64_t)al * a.stride;
-    if (c < 0) {  // tracker life-time limit: identity row
-      fs[al] = src[i];
-      continue;
-    }
-    T lx[P + 1], ly[P + 1], lz[P + 1];
-    lagrange_all<P, T>(a.gll, a.inv_den, (T)a.xi[r * D + 0], lx);
-    lagrange_all<P, T>(a.gll, a.inv_den, (T)a.xi[r * D + 1], ly);
-    if constexpr (D == 3) lagrange_all<P, T>(a.gll, a.inv_den, (T)a.xi[r * D + 2], lz);
-    const int cx = c % a.N[0];
-    const int cy = (c / a.N[0]) % a.N[1];
-    const int cz = (D == 3) ? c / (a.N[0] * a.N[1]) : 0;
-    const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P + (int64_t)n1 * ((int64_t)cz * P));
-    T acc = 0;
-    if constexpr (D == 2) {
-#pragma unroll
-      for (int my = 0; my <= P; my++)
-#pragma unroll
-        for (int mx = 0; mx <= P; mx++) {
-          const T w = lx[mx] * ly[my];
-          if (!(fabs(w) < T(1e-10))) acc += w * src[base + mx + (int64_t)n0 * my];
-        }
-    } else {
-#pragma unroll
-      for (int mz = 0; mz <= P; mz++)
+    if constexpr (sizeof(T) == 8) {
+      const int c = a.cell[r];
+      if (c < 0) {  // tracker life-time limit: identity row
+        fs[al] = src[i];
+        continue;
+      }
+      T lx[P + 1], ly[P + 1], lz[P + 1];
+      lagrange_all<P, T>(a.gll, a.inv_den, (T)a.xi[r * D + 0], lx);
+      lagrange_all<P, T>(a.gll, a.inv_den, (T)a.xi[r * D + 1], ly);
+      if constexpr (D == 3) lagrange_all<P, T>(a.gll, a.inv_den, (T)a.xi[r * D + 2], lz);
+      const int cx = c % a.N[0];
+      const int cy = (c / a.N[0]) % a.N[1];
+      const int cz = (D == 3) ? c / (a.N[0] * a.N[1]) : 0;
+      const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P + (int64_t)n1 * ((int64_t)cz * P));
+      T acc = 0;
+      if constexpr (D == 2) {
 #pragma unroll
         for (int my = 0; my <= P; my++)
 #pragma unroll
           for (int mx = 0; mx <= P; mx++) {
-            const T w = lx[mx] * ly[my] * lz[mz];
-            if (!(fabs(w) < T(1e-10))) acc += w * src[base + mx + (int64_t)n0 * (my + (int64_t)n1 * mz)];
+            const T w = lx[mx] * ly[my];
+            if (!(fabs(w) < T(1e-10))) acc += w * src[base + mx + (int64_t)n0 * my];
           }
+      } else {
+#pragma unroll
+        for (int mz = 0; mz <= P; mz++)
+#pragma unroll
+          for (int my = 0; my <= P; my++)
+#pragma unroll
+            for (int mx = 0; mx <= P; mx++) {
+              const T w = lx[mx] * ly[my] * lz[mz];
+              if (!(fabs(w) < T(1e-10))) acc += w * src[base + mx + (int64_t)n0 * (my + (int64_t)n1 * mz)];
+            }
+      }
+      fs[al] = acc;
+    } else {
+      // FP32: weights from the anchor offsets, delta form around the anchor node
+      // (its own term is w_a (f_a - f_a) = 0, so it needs no special case)
+      const int code = a.code[r];
+      if (code < 0) {
+        fs[al] = src[i];
+        continue;
+      }
+      constexpr int NP = P + 1;
+      constexpr int NL = (D == 3) ? NP * NP * NP : NP * NP;
+      const int c = code / NL;
+      const int lin = code - c * NL;
+      const int ax = lin % NP, ay = (lin / NP) % NP, az = (D == 3) ? lin / (NP * NP) : 0;
+      T lx[NP], ly[NP], lz[NP];
+      auto basis = [&](int anc, T d, T(&v)[NP]) {
+#pragma unroll
+        for (int j = 0; j <= P; j++) {
+          T t = a.inv_den[j];
+#pragma unroll
+          for (int k = 0; k <= P; k++)
+            if (k != j) t *= (a.dgl[anc][k] + d);
+          v[j] = t;
+        }
+      };
+      basis(ax, (T)a.xi[r * D + 0], lx);
+      basis(ay, (T)a.xi[r * D + 1], ly);
+      if constexpr (D == 3) basis(az, (T)a.xi[r * D + 2], lz);
+      const int cx = c % a.N[0];
+      const int cy = (c / a.N[0]) % a.N[1];
+      const int cz = (D == 3) ? c / (a.N[0] * a.N[1]) : 0;
+      const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P + (int64_t)n1 * ((int64_t)cz * P));
+      const T fa = src[base + ax + (int64_t)n0 * (ay + (int64_t)n1 * az)];
+      T acc = 0;
+      if constexpr (D == 2) {
+#pragma unroll
+        for (int my = 0; my <= P; my++)
+#pragma unroll
+          for (int mx = 0; mx <= P; mx++) {
+            const T w = lx[mx] * ly[my];
+            if (!(fabs(w) < T(1e-10))) acc += w * (src[base + mx + (int64_t)n0 * my] - fa);
+          }
+      } else {
+#pragma unroll
+        for (int mz = 0; mz <= P; mz++)
+#pragma unroll
+          for (int my = 0; my <= P; my++)
+#pragma unroll
+            for (int mx = 0; mx <= P; mx++) {
+              const T w = lx[mx] * ly[my] * lz[mz];
+              if (!(fabs(w) < T(1e-10))) acc += w * (src[base + mx + (int64_t)n0 * (my + (int64_t)n1 * mz)] - fa);
+            }
+      }
+      fs[al] = fa + acc;
     }
-    fs[al] = acc;
   }
   bgk_collide<Q, T>(fs, ca, i, write_macro != 0);
 #pragma unroll
@@ -136,6 +193,7 @@ static void fill_general(nat_ctx* ctx, GeneralArgs<T, XT>& a, const XT* xi) {
     a.N[d] = d < ctx->D ? ctx->desc.ncells[d] : 1;
   }
   a.cell = ctx->dep_cell;
+  a.code = ctx->dep_code32;
   a.xi = xi;
   double x[MAXP + 1], w[MAXP + 1];
   gll_points(ctx->p, x, w);
@@ -146,6 +204,8 @@ static void fill_general(nat_ctx* ctx, GeneralArgs<T, XT>& a, const XT* xi) {
       if (m != j && j <= ctx->p) d *= (x[j] - x[m]);
     a.inv_den[j] = j <= ctx->p ? (T)(1.0 / d) : T(0);
   }
+  for (int j = 0; j <= MAXP; j++)
+    for (int k = 0; k <= MAXP; k++) a.dgl[j][k] = (j <= ctx->p && k <= ctx->p) ? (T)(x[j] - x[k]) : T(0);
 }
 
 }  // namespace nat
@@ -159,6 +219,7 @@ int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
     fill_general<float, float>(ctx, a, ctx->dep_xi32);
     CollideArgs<float> ca;
     ca.omega = (float)(1.0 / ctx->tau);
+    ca.omega64 = 1.0 / ctx->tau;
     ca.rho0 = (float)(ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0);
     ca.scaling = ctx->st.s;
     ca.rho = (float*)ctx->rho;
@@ -171,6 +232,7 @@ int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
     fill_general<double, double>(ctx, a, ctx->dep_xi64);
     CollideArgs<double> ca;
     ca.omega = 1.0 / ctx->tau;
+    ca.omega64 = 1.0 / ctx->tau;
     ca.rho0 = 0.0;
     ca.scaling = ctx->st.s;
     ca.rho = (double*)ctx->rho;

@@ -4,16 +4,40 @@
 // (advection/SemiLagrangian.cpp:145-522) is a tensor product of 1-D operators:
 // the departure point of node k along axis a lies in cell floor(k/p) + o[j] at a
 // unit coordinate that only depends on the node's position j = k mod p in its cell
-// and on the velocity component. Every row of A_alpha is therefore
+// and on the sign of the velocity component. Every row of A_alpha is therefore
 //   f_alpha(x_k) = sum_{m,n,l} Wx[jx][m] Wy[jy][n] Wz[jz][l] f_alpha(src nodes)
-// and is applied here by sum factorisation: per direction the source window of
-// a tile is staged in LDS once and interpolated axis by axis (x, then y, then z),
-// so each population is read from HBM once per step and written once.
+// and is applied by sum factorisation, one 1-D pass per non-zero velocity
+// component, in the order z, y, x.
+//
+// Work shapes. A block owns a tile of TX*TY*TZ output nodes, one thread per node
+// for the collision. In a 1-D pass a work item owns one source cell of one line:
+// it reads the cell's p+1 values once and produces the p outputs that depart from
+// that cell (for a positive component the nodes 1..p of the cell, the last one
+// being the next cell's vertex; otherwise the nodes 0..p-1). The first pass of a
+// direction reads its source window straight from global memory (L2-resident
+// halos); later passes read the previous pass's output from LDS; the last pass
+// writes the tile's values, which each thread picks up for its own node.
+//
+// Scheduling. Directions are processed in groups of B, ordered by their number of
+// passes so that a group needs as few barriers as its longest member: all first
+// passes of a group, one barrier, all second passes, one barrier, ... The global
+// loads of the next group's first passes are issued right after the current
+// group's first passes, so they are in flight during the remaining passes
+// (barriers here wait for LDS traffic only).
+//
+// Arithmetic. Interpolation weights, pass accumulations, LDS intermediates and
+// the collision run in FP64 for both storage precisions (non-packed FP32 and FP64
+// FMAs have the same rate on CDNA4, and FP64 intermediates need no conversions
+// between passes). With FP32 storage the populations are shifted,
+// f~ = f - w rho0, so only perturbations are rounded to FP32.
 //
 // Storage: unique periodic nodes (N_a p per axis, x fastest); the FE_Q DoFs on the
 // upper periodic faces are copies of the lower ones and are produced at download.
-// Precision T = double, or float on shifted populations f - w rho0.
 #include "nat_collide.hpp"
+
+#ifndef NAT_STRUCT_B
+#define NAT_STRUCT_B 3  // directions per group
+#endif
 
 namespace nat {
 
@@ -23,173 +47,290 @@ struct StructArgs {
   T* fout;
   int64_t stride;
   int n[3];
-  const T* w;      // [3][MAXSHIFT][MAXP][MAXP+1]
+  int tiles[3];       // tiles per axis
+  int xcd_remap;      // 1: group spatially adjacent tiles on the same XCD
+  const double* w;    // [3][MAXSHIFT][MAXP][MAXP+1] FP64 weights
 };
 
-// Source-cell pattern of a velocity component (checked at assembly time):
-//   component > 0 (departure towards -x): the vertex node (j = 0) reads the cell
-//   below, nodes j >= 1 read their own cell; component < 0: every node reads its
-//   own cell. So the source window of a cell-aligned tile [X0, X0+T) is
-//   [X0-P, X0+T] (component > 0), [X0, X0+T] (< 0) or [X0, X0+T) (= 0).
-template <int C, int T_, int P>
-struct Win {
-  static constexpr int lo = C > 0 ? -P : 0;
-  static constexpr int n = C > 0 ? T_ + P + 1 : (C < 0 ? T_ + 1 : T_);
+// Source window of a tile along one axis for a velocity component c (pattern
+// checked at assembly): c > 0 -> nodes [X0-P, X0+T] (the vertex node reads the
+// cell below), c < 0 -> [X0, X0+T], c = 0 -> [X0, X0+T).
+constexpr int win_n(int c, int t, int p) { return c > 0 ? t + p + 1 : (c < 0 ? t + 1 : t); }
+constexpr int win_lo(int c, int p) { return c > 0 ? -p : 0; }
+
+// Compile-time geometry of pass K of direction AL.
+template <int D, int Q, int P, int TX, int TY, int TZ, int AL, int K>
+struct PassGeo {
+  using L = Lattice<Q>;
+  static constexpr int c0 = L::e[AL][0], c1 = L::e[AL][1], c2 = (D == 3) ? L::e[AL][2] : 0;
+  static constexpr int comp(int a) { return a == 0 ? c0 : (a == 1 ? c1 : c2); }
+  static constexpr int tile(int a) { return a == 0 ? TX : (a == 1 ? TY : TZ); }
+  static constexpr int npass = (c0 != 0) + (c1 != 0) + (c2 != 0);
+  static constexpr int axis_of(int k) {
+    int n = 0;
+    for (int a = 2; a >= 0; a--)
+      if (comp(a) != 0) {
+        if (n == k) return a;
+        n++;
+      }
+    return -1;
+  }
+  static constexpr bool processed(int a) {
+    for (int j = 0; j < K; j++)
+      if (axis_of(j) == a) return true;
+    return false;
+  }
+  static constexpr int A = axis_of(K);
+  static constexpr int C = comp(A);
+  static constexpr bool first = (K == 0), last = (K == npass - 1);
+  static constexpr int ext(int a) { return processed(a) ? tile(a) : win_n(comp(a), tile(a), P); }
+  static constexpr int oext(int a) { return a == A ? tile(A) : ext(a); }
+  static constexpr int EX = ext(0), EY = ext(1), EZ = ext(2);
+  static constexpr int OX = oext(0), OY = oext(1), OZ = oext(2);
+  static constexpr int NC = C > 0 ? tile(A) / P + 1 : tile(A) / P;  // source cells covering the tile
+  static constexpr int O1 = (A == 0) ? 1 : 0, O2 = (A == 2) ? 1 : 2;  // other axes, O1 < O2
+  static constexpr int E1 = ext(O1), E2 = ext(O2);
+  static constexpr int items = E1 * E2 * NC;
 };
 
-constexpr int pow2ceil(int v) {
-  int r = 1;
-  while (r < v) r <<= 1;
-  return r;
+// Processing order: directions grouped B at a time, groups never mixing
+// directions with different numbers of passes.
+template <int D, int Q, int B>
+struct Schedule {
+  int ngroups = 0;
+  int dir[32][4] = {};
+  int npass[32] = {};
+  constexpr Schedule() {
+    for (int cls = 1; cls <= 3; cls++) {
+      int slot = B;
+      for (int al = 1; al < Q; al++) {
+        const int np = (Lattice<Q>::e[al][0] != 0) + (Lattice<Q>::e[al][1] != 0) +
+                       ((D == 3) ? (Lattice<Q>::e[al][2] != 0) : 0);
+        if (np != cls) continue;
+        if (slot == B) {
+          ngroups++;
+          slot = 0;
+          npass[ngroups - 1] = cls;
+        }
+        dir[ngroups - 1][slot++] = al;
+      }
+    }
+  }
+};
+template <int D, int Q, int B>
+inline constexpr Schedule<D, Q, B> kSchedule{};
+
+// p outputs of one source cell. v: the cell's p+1 values. Output kk is the node at
+// position k = kk + (C > 0) of the source cell; its class is k mod p. Weights are
+// FP64 rows [class][m] (row pitch MAXP+1) read with uniform addresses.
+template <int C, int P>
+__device__ __forceinline__ void cell_outputs(const double* __restrict__ wt, const double (&v)[P + 1],
+                                             double (&out)[P]) {
+#pragma unroll
+  for (int kk = 0; kk < P; kk++) {
+    const int j = (kk + (C > 0 ? 1 : 0)) % P;
+    double acc = 0;
+#pragma unroll
+    for (int m = 0; m <= P; m++) acc += wt[j * (MAXP + 1) + m] * v[m];
+    out[kk] = acc;
+  }
 }
 
-template <int D, int Q, int P, typename T, int TX, int TY, int TZ>
-__global__ __launch_bounds__(TX* TY* TZ) void k_sl_struct(StructArgs<T> a, CollideArgs<T> ca, int write_macro) {
-  using L = Lattice<Q>;
+// Barrier for LDS hand-over only: outstanding global loads stay in flight.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <int D, int Q, int P, int TX, int TY, int TZ, int B>
+struct StructCfg {
+  static constexpr int NT = TX * TY * TZ;
+  static constexpr int WX = TX + P + 1, WY = TY + P + 1, WZ = (D == 3) ? TZ + P + 1 : 1;
+  // first-pass outputs: z done (x, y windows) in 3D, y done in 2D
+  static constexpr int B1 = (D == 3) ? WX * WY * TZ : WX * TY;
+  static constexpr int B2 = (D == 3) ? WX * TY * TZ : 1;
+  static constexpr int PX = TX + 1;  // padded row pitch of the tile output
+  static constexpr int B3 = PX * TY * TZ;
+  // register slots for a first pass: worst case over the pass shapes
+  static constexpr int max3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
+  static constexpr int FIRST_ITEMS =
+      (D == 3) ? max3(WX * WY * (TZ / P + 1), WX * (TY / P + 1) * TZ, (TX / P + 1) * TY * TZ)
+               : max3(WX * (TY / P + 1), (TX / P + 1) * TY, 1);
+  static constexpr int KI = (FIRST_ITEMS + NT - 1) / NT;
+};
+
+template <int D, int Q, int P, typename T, int TX, int TY, int TZ, int B, int WPE>
+__global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_sl_struct(
+    StructArgs<T> a, const double* __restrict__ wtab, CollideArgs<T> ca, int write_macro) {
+  using Cfg = StructCfg<D, Q, P, TX, TY, TZ, B>;
   static_assert(TX % P == 0 && TY % P == 0 && (D == 2 || TZ % P == 0), "tiles must be cell aligned");
-  constexpr int NT = TX * TY * TZ;
-  constexpr int WX = TX + P + 1, WY = TY + P + 1, WZ = (D == 3) ? TZ + P + 1 : 1;
-  constexpr int RW = pow2ceil(WX);  // lanes per staged row
-  static_assert(NT % TX == 0, "block must tile the x-pass rows");
-  __shared__ T win[WX * WY * WZ];
-  __shared__ T b1[TX * WY * WZ];
-  __shared__ T b2[(D == 3) ? TX * TY * WZ : 1];
-  __shared__ T wt_s[3 * 2 * P * (P + 1)];  // [axis][shift 0:+c 1:-c][class][m]
+  constexpr int NT = Cfg::NT, KI = Cfg::KI;
+  constexpr int TW = (TX > TY ? (TX > TZ ? TX : TZ) : (TY > TZ ? TY : TZ)) + P + 1;
+  __shared__ double b1[B][Cfg::B1];
+  __shared__ double b2[B][Cfg::B2];
+  __shared__ T b3[2][B][Cfg::B3];
+  __shared__ int gt[3][TW];  // premultiplied wrapped offsets of window nodes [X0-P, X0+T] per axis
 
   const int tid = threadIdx.x;
-  for (int t = tid; t < 3 * 2 * P * (P + 1); t += NT) {
-    const int ax = t / (2 * P * (P + 1)), sh = (t / (P * (P + 1))) % 2, j = (t / (P + 1)) % P, m = t % (P + 1);
-    wt_s[t] = a.w[((ax * MAXSHIFT + sh) * MAXP + j) * (MAXP + 1) + m];
-  }
-  __syncthreads();
-  const int tid0 = tid;
-  const int X00 = blockIdx.x * TX, Y00 = blockIdx.y * TY, Z00 = blockIdx.z * TZ;
+  // tile index, optionally regrouped so that consecutive tiles (which share halos)
+  // run on the same XCD: hardware dispatches workgroup i to XCD i % 8
+  int bid = blockIdx.x;
+  const int nb = a.tiles[0] * a.tiles[1] * a.tiles[2];
+  if (a.xcd_remap) bid = (bid % 8) * (nb / 8) + bid / 8;
+  const int org[3] = {(bid % a.tiles[0]) * TX, ((bid / a.tiles[0]) % a.tiles[1]) * TY,
+                      (bid / (a.tiles[0] * a.tiles[1])) * TZ};
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
-  const bool active = (X00 + tid % TX < nx) && (Y00 + (tid / TX) % TY < ny) && (Z00 + tid / (TX * TY) < nz);
-  const int64_t node =
-      (int64_t)(X00 + tid % TX) + (int64_t)nx * ((Y00 + (tid / TX) % TY) + (int64_t)ny * (Z00 + tid / (TX * TY)));
+  const int astr[3] = {1, nx, nx * ny};
+  const int alim[3] = {nx, nx * ny, nx * ny * nz};  // n_a * stride_a
+  for (int i = tid; i < 3 * TW; i += NT) {
+    const int ax = i / TW, k = i % TW;
+    const int n = a.n[ax];
+    gt[ax][k] = (ax < D) ? (((org[ax] - P + k) % n + n) % n) * astr[ax] : 0;
+  }
+  const int ox0 = tid % TX, oy0 = (tid / TX) % TY, oz0 = tid / (TX * TY);
+  const bool active = (org[0] + ox0 < nx) && (org[1] + oy0 < ny) && (org[2] + oz0 < nz);
+  const int node = (org[0] + ox0) + nx * ((org[1] + oy0) + ny * (org[2] + oz0));
+  const int myb3 = ox0 + Cfg::PX * (oy0 + TY * oz0);
 
   T fs[Q];
   fs[0] = active ? a.fin[node] : T(0);
+  T xin[B][KI][P + 1];
+  __syncthreads();
 
-  static_for<1, Q>([&](auto A) {
-    constexpr int al = decltype(A)::value;
-    constexpr int ex = L::e[al][0], ey = L::e[al][1], ez = (D == 3) ? L::e[al][2] : 0;
-    using Wx = Win<ex, TX, P>;
-    using Wy = Win<ey, TY, P>;
-    using Wz = Win<ez, (D == 3 ? TZ : 1), (D == 3 ? P : 0)>;
-    constexpr int wnx = Wx::n, wny = Wy::n, wnz = (D == 3) ? Wz::n : 1;
-    const T* __restrict__ src = a.fin + (int64_t)al * a.stride;
-    // opaque copies: stop the compiler from sharing index arithmetic across
-    // directions (it would keep every direction's addresses live at once)
-    int tid = tid0, X0 = X00, Y0 = Y00, Z0 = Z00;
-    asm volatile("" : "+v"(tid), "+s"(X0), "+s"(Y0), "+s"(Z0));
-    const int ox = tid % TX, oy = (tid / TX) % TY, oz = tid / (TX * TY);
-    const int jx = ox % P, jy = oy % P, jz = oz % P;
-    const int wlx = X0 + Wx::lo, wly = Y0 + Wy::lo, wlz = Z0 + ((D == 3) ? Wz::lo : 0);
-
-    // stage the window with periodic wrap; RW (power of two) slots per row
-    {
-      constexpr int rows = wny * wnz;
-      for (int t = tid; t < rows * RW; t += NT) {
-        const int ix = t % RW, r = t / RW;
-        if (ix >= wnx) continue;
-        const int iy = r % wny, iz = r / wny;
-        // windows start at most P nodes below 0 and P <= n, so +n makes the
-        // argument non-negative; the modulo also covers tiles wider than the axis
-        const int gx = (wlx + ix + nx) % nx;
-        const int gy = (wly + iy + ny) % ny;
-        const int gz = (D == 3) ? (wlz + iz + nz) % nz : 0;
-        win[ix + WX * (iy + WY * iz)] = src[(int64_t)gx + (int64_t)nx * (gy + (int64_t)ny * gz)];
-      }
-    }
-    __syncthreads();
-
-    // x pass: b1[ox][iy][iz] (win itself when ex == 0)
-    if constexpr (ex != 0) {
-      const T* wt = wt_s + ((0 * 2 + (ex > 0 ? 0 : 1)) * P + jx) * (P + 1);
-      T wv[P + 1];
+  // ---- global loads of a direction's first pass into register slot b
+  auto load_first = [&](auto ALc, auto Bc) {
+    constexpr int al = decltype(ALc)::value, b = decltype(Bc)::value;
+    if constexpr (al > 0) {
+      using G = PassGeo<D, Q, P, TX, TY, TZ, al, 0>;
+      constexpr int KK = (G::items + NT - 1) / NT;
+      static_assert(KK <= KI, "first-pass register slots");
+      const T* __restrict__ src = a.fin + (int64_t)al * a.stride;
+      int t0 = tid;
+      asm volatile("" : "+v"(t0));
 #pragma unroll
-      for (int m = 0; m <= P; m++) wv[m] = wt[m];
-      // source cell start inside the window
-      const int s0 = (ox / P) * P + ((ex > 0 && jx != 0) ? P : 0);
-      constexpr int rows = wny * wnz;
-      for (int r = tid / TX; r < rows; r += NT / TX) {
-        const int iy = r % wny, iz = r / wny;
-        const T* wr = win + WX * (iy + WY * iz) + s0;
-        T acc = 0;
+      for (int k = 0; k < KK; k++) {
+        const int it = t0 + k * NT;
+        if (it < G::items) {
+          const int i1 = it % G::E1, r = it / G::E1;
+          const int i2 = r % G::E2, sc = r / G::E2;
+          const int base = gt[G::O1][i1 + win_lo(G::comp(G::O1), P) + P] +
+                           gt[G::O2][i2 + win_lo(G::comp(G::O2), P) + P];
+          const int g0 = gt[G::A][sc * P + win_lo(G::C, P) + P];
 #pragma unroll
-        for (int m = 0; m <= P; m++) acc += wv[m] * wr[m];
-        b1[ox + TX * (iy + WY * iz)] = acc;
-      }
-      __syncthreads();
-    }
-    const T* B1 = (ex != 0) ? b1 : win;
-    constexpr int b1x = (ex != 0) ? TX : WX;
-
-    T v;
-    if constexpr (D == 2) {
-      if constexpr (ey != 0) {
-        const T* wt = wt_s + ((1 * 2 + (ey > 0 ? 0 : 1)) * P + jy) * (P + 1);
-        const int s0 = (oy / P) * P + ((ey > 0 && jy != 0) ? P : 0);
-        T acc = 0;
-#pragma unroll
-        for (int n = 0; n <= P; n++) acc += wt[n] * B1[ox + b1x * (s0 + n)];
-        v = acc;
-      } else {
-        v = B1[ox + b1x * oy];
-      }
-    } else {
-      const T* B2 = B1;
-      constexpr int b2x = (ey != 0) ? TX : b1x;
-      constexpr int b2y = (ey != 0) ? TY : WY;
-      if constexpr (ey != 0) {
-        const T* wt = wt_s + ((1 * 2 + (ey > 0 ? 0 : 1)) * P + jy) * (P + 1);
-        T wv[P + 1];
-#pragma unroll
-        for (int n = 0; n <= P; n++) wv[n] = wt[n];
-        const int s0 = (oy / P) * P + ((ey > 0 && jy != 0) ? P : 0);
-#pragma unroll
-        for (int iz = oz; iz < wnz; iz += TZ) {
-          const T* br = B1 + ox + b1x * (s0 + WY * iz);
-          T acc = 0;
-#pragma unroll
-          for (int n = 0; n <= P; n++) acc += wv[n] * br[b1x * n];
-          b2[ox + TX * (oy + TY * iz)] = acc;
+          for (int m = 0; m <= P; m++) {
+            int g = g0 + m * astr[G::A];
+            g -= (g >= alim[G::A]) ? alim[G::A] : 0;
+            xin[b][k][m] = src[base + g];
+          }
         }
-        __syncthreads();
-        B2 = b2;
-      }
-      if constexpr (ez != 0) {
-        const T* wt = wt_s + ((2 * 2 + (ez > 0 ? 0 : 1)) * P + jz) * (P + 1);
-        const int s0 = (oz / P) * P + ((ez > 0 && jz != 0) ? P : 0);
-        T acc = 0;
-#pragma unroll
-        for (int l = 0; l <= P; l++) acc += wt[l] * B2[ox + b2x * (oy + b2y * (s0 + l))];
-        v = acc;
-      } else {
-        v = B2[ox + b2x * (oy + b2y * oz)];
       }
     }
-    // materialise this direction's value now (otherwise its operands stay live
-    // until the collision)
-    asm volatile("" : "+v"(v));
-    fs[al] = v;
-    __syncthreads();
-    asm volatile("" ::: "memory");
+  };
+
+  // ---- one 1-D pass of direction al (slot b, output parity par)
+  auto run_pass = [&](auto ALc, auto Kc, auto Bc, auto PARc) {
+    constexpr int al = decltype(ALc)::value, kp = decltype(Kc)::value;
+    constexpr int b = decltype(Bc)::value, par = decltype(PARc)::value;
+    using G = PassGeo<D, Q, P, TX, TY, TZ, al, kp>;
+    constexpr int KK = (G::items + NT - 1) / NT;
+    const double* wt = wtab + (G::A * MAXSHIFT + (G::C > 0 ? 0 : 1)) * MAXP * (MAXP + 1);
+    constexpr int istr = (G::A == 0) ? 1 : (G::A == 1 ? G::EX : G::EX * G::EY);  // input stride along A
+    int t0 = tid;
+    asm volatile("" : "+v"(t0));
+#pragma unroll
+    for (int k = 0; k < KK; k++) {
+      const int it = t0 + k * NT;
+      if (it < G::items) {
+        const int i1 = it % G::E1, r = it / G::E1;
+        const int i2 = r % G::E2, sc = r / G::E2;
+        int c[3];
+        c[G::O1] = i1;
+        c[G::O2] = i2;
+        c[G::A] = sc * P;
+        double v[P + 1], o[P];
+        if constexpr (G::first) {
+#pragma unroll
+          for (int m = 0; m <= P; m++) v[m] = (double)xin[b][k][m];
+        } else {
+          const double* in = (kp == 1) ? b1[b] : b2[b];
+          const int i0 = c[0] + G::EX * (c[1] + G::EY * c[2]);
+#pragma unroll
+          for (int m = 0; m <= P; m++) v[m] = in[i0 + m * istr];
+        }
+        cell_outputs<G::C, P>(wt, v, o);
+        constexpr int sh = (G::C > 0) ? 1 - P : 0;
+#pragma unroll
+        for (int kk = 0; kk < P; kk++) {
+          const int oa = sc * P + kk + sh;
+          if (oa >= 0 && oa < G::tile(G::A)) {
+            int d[3] = {c[0], c[1], c[2]};
+            d[G::A] = oa;
+            if constexpr (G::last) {
+              b3[par][b][d[0] + Cfg::PX * (d[1] + TY * d[2])] = (T)o[kk];
+            } else {
+              double* out = (kp == 0) ? b1[b] : b2[b];
+              out[d[0] + G::OX * (d[1] + G::OY * d[2])] = o[kk];
+            }
+          }
+        }
+      }
+    }
+  };
+
+  constexpr int NG = kSchedule<D, Q, B>.ngroups;
+  static_for<0, B>([&](auto Bc) {
+    constexpr int b = decltype(Bc)::value;
+    load_first(std::integral_constant<int, kSchedule<D, Q, B>.dir[0][b]>{}, Bc);
+  });
+
+  static_for<0, NG>([&](auto Gc) {
+    constexpr int g = decltype(Gc)::value;
+    constexpr int np = kSchedule<D, Q, B>.npass[g];
+    using PAR = std::integral_constant<int, g & 1>;
+    // first passes (register inputs)
+    static_for<0, B>([&](auto Bc) {
+      constexpr int al = kSchedule<D, Q, B>.dir[g][decltype(Bc)::value];
+      if constexpr (al > 0) run_pass(std::integral_constant<int, al>{}, std::integral_constant<int, 0>{}, Bc, PAR{});
+    });
+    // next group's inputs are in flight during the remaining passes
+    if constexpr (g + 1 < NG) {
+      static_for<0, B>([&](auto Bc) {
+        constexpr int b = decltype(Bc)::value;
+        load_first(std::integral_constant<int, kSchedule<D, Q, B>.dir[g + 1][b]>{}, Bc);
+      });
+    }
+    lds_barrier();
+    static_for<1, np>([&](auto Kc) {
+      static_for<0, B>([&](auto Bc) {
+        constexpr int al = kSchedule<D, Q, B>.dir[g][decltype(Bc)::value];
+        if constexpr (al > 0) run_pass(std::integral_constant<int, al>{}, Kc, Bc, PAR{});
+      });
+      lds_barrier();
+    });
+    // pick up this group's values for the own node
+    static_for<0, B>([&](auto Bc) {
+      constexpr int b = decltype(Bc)::value;
+      constexpr int al = kSchedule<D, Q, B>.dir[g][b];
+      if constexpr (al > 0) {
+        T val = b3[g & 1][b][myb3];
+        asm volatile("" : "+v"(val));
+        fs[al] = val;
+      }
+    });
   });
 
   if (!active) return;
-  bgk_collide<Q, T>(fs, ca, node, write_macro != 0);
-#pragma unroll
-  for (int i = 0; i < Q; i++) a.fout[(int64_t)i * a.stride + node] = fs[i];
+  bgk_collide_store<Q, T>(fs, ca, node, write_macro != 0, a.fout, a.stride);
 }
 
 template <int D, int Q, int P, typename T, int TX, int TY, int TZ>
-static int launch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideArgs<T>& ca, int write_macro) {
-  dim3 grid((a.n[0] + TX - 1) / TX, (a.n[1] + TY - 1) / TY, D == 3 ? (a.n[2] + TZ - 1) / TZ : 1);
-  hipLaunchKernelGGL((k_sl_struct<D, Q, P, T, TX, TY, TZ>), grid, dim3(TX * TY * TZ), 0, ctx->stream, a, ca,
-                     write_macro);
+static int launch_struct(nat_ctx* ctx, StructArgs<T> a, const CollideArgs<T>& ca, int write_macro) {
+  a.tiles[0] = (a.n[0] + TX - 1) / TX;
+  a.tiles[1] = (a.n[1] + TY - 1) / TY;
+  a.tiles[2] = D == 3 ? (a.n[2] + TZ - 1) / TZ : 1;
+  const int nb = a.tiles[0] * a.tiles[1] * a.tiles[2];
+  a.xcd_remap = (nb % 8 == 0) ? 1 : 0;
+  constexpr int B = NAT_STRUCT_B;
+  constexpr int WPE = 4;  // 16 waves per CU: one 1024-thread tile, up to 128 VGPRs
+  hipLaunchKernelGGL((k_sl_struct<D, Q, P, T, TX, TY, TZ, B, WPE>), dim3(nb), dim3(TX * TY * TZ), 0, ctx->stream,
+                     a, a.w, ca, write_macro);
   NAT_HIP(hipGetLastError());
   return 0;
 }
@@ -202,22 +343,22 @@ static int dispatch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideAr
 #define NAT_S2(QQ, PP, TX, TY) \
   if (Q == QQ && P == PP) return launch_struct<2, QQ, PP, T, TX, TY, 1>(ctx, a, ca, wm);
   if (ctx->D == 3) {
-    NAT_S3(27, 1, 16, 8, 4)
-    NAT_S3(27, 2, 16, 8, 4)
-    NAT_S3(27, 3, 12, 6, 6)
-    NAT_S3(27, 4, 16, 8, 4)
-    NAT_S3(27, 6, 12, 6, 6)
-    NAT_S3(19, 1, 16, 8, 4)
-    NAT_S3(19, 2, 16, 8, 4)
-    NAT_S3(19, 3, 12, 6, 6)
-    NAT_S3(19, 4, 16, 8, 4)
-    NAT_S3(19, 6, 12, 6, 6)
+    NAT_S3(27, 1, 16, 8, 8)
+    NAT_S3(27, 2, 16, 8, 8)
+    NAT_S3(27, 3, 12, 12, 6)
+    NAT_S3(27, 4, 16, 8, 8)
+    NAT_S3(27, 6, 12, 12, 6)
+    NAT_S3(19, 1, 16, 8, 8)
+    NAT_S3(19, 2, 16, 8, 8)
+    NAT_S3(19, 3, 12, 12, 6)
+    NAT_S3(19, 4, 16, 8, 8)
+    NAT_S3(19, 6, 12, 12, 6)
   } else {
     NAT_S2(9, 1, 32, 16)
     NAT_S2(9, 2, 32, 16)
-    NAT_S2(9, 3, 24, 18)
+    NAT_S2(9, 3, 24, 24)
     NAT_S2(9, 4, 32, 16)
-    NAT_S2(9, 6, 24, 18)
+    NAT_S2(9, 6, 24, 24)
   }
 #undef NAT_S3
 #undef NAT_S2
@@ -225,45 +366,31 @@ static int dispatch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideAr
   return NAT_ERR_UNSUPPORTED;
 }
 
-}  // namespace nat
-
-int nat_launch_sl_struct(nat_ctx* ctx, int write_macro) {
-  using namespace nat;
+template <typename T>
+static int launch_typed(nat_ctx* ctx, int write_macro) {
   const int src = ctx->cur, dst = 1 - ctx->cur;
-  if (ctx->fp32) {
-    StructArgs<float> a;
-    a.fin = (const float*)ctx->f[src];
-    a.fout = (float*)ctx->f[dst];
-    a.stride = ctx->stride;
-    for (int i = 0; i < 3; i++) a.n[i] = ctx->nu[i];
-    a.w = (const float*)ctx->ax_w;
-    CollideArgs<float> ca;
-    ca.omega = (float)(1.0 / ctx->tau);
-    ca.rho0 = (float)(ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0);
-    ca.scaling = ctx->st.s;
-    ca.rho = (float*)ctx->rho;
-    ca.u = (float*)ctx->u;
-    ca.mstride = ctx->n_store;
-    ca.status = ctx->d_status;
-    int rc = dispatch_struct<float>(ctx, a, ca, write_macro);
-    if (rc == 0) ctx->cur = dst;
-    return rc;
-  }
-  StructArgs<double> a;
-  a.fin = (const double*)ctx->f[src];
-  a.fout = (double*)ctx->f[dst];
+  StructArgs<T> a;
+  a.fin = (const T*)ctx->f[src];
+  a.fout = (T*)ctx->f[dst];
   a.stride = ctx->stride;
   for (int i = 0; i < 3; i++) a.n[i] = ctx->nu[i];
   a.w = (const double*)ctx->ax_w;
-  CollideArgs<double> ca;
-  ca.omega = 1.0 / ctx->tau;
-  ca.rho0 = 0.0;
+  CollideArgs<T> ca;
+  ca.omega = (T)(1.0 / ctx->tau);
+  ca.omega64 = 1.0 / ctx->tau;
+  ca.rho0 = (T)(ctx->fp32 ? (ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0) : 0.0);
   ca.scaling = ctx->st.s;
-  ca.rho = (double*)ctx->rho;
-  ca.u = (double*)ctx->u;
+  ca.rho = (T*)ctx->rho;
+  ca.u = (T*)ctx->u;
   ca.mstride = ctx->n_store;
   ca.status = ctx->d_status;
-  int rc = dispatch_struct<double>(ctx, a, ca, write_macro);
+  int rc = dispatch_struct<T>(ctx, a, ca, write_macro);
   if (rc == 0) ctx->cur = dst;
   return rc;
+}
+
+}  // namespace nat
+
+int nat_launch_sl_struct(nat_ctx* ctx, int write_macro) {
+  return ctx->fp32 ? nat::launch_typed<float>(ctx, write_macro) : nat::launch_typed<double>(ctx, write_macro);
 }

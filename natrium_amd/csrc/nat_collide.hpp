@@ -69,4 +69,104 @@ __device__ __forceinline__ void bgk_collide(T (&f)[Q], const CollideArgs<T>& ca,
   }
 }
 
+// Same collision with FP64 arithmetic on values held in FP64 registers, for any
+// storage type T (FP32 storage keeps the shift by rho0; FP64 storage has rho0 = 0).
+template <int Q, typename T>
+__device__ __forceinline__ void bgk_collide_f64(double (&f)[Q], const CollideArgs<T>& ca, int64_t node,
+                                                bool write_macro) {
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  double drho = 0;
+#pragma unroll
+  for (int i = 0; i < Q; i++) drho += f[i];
+  double j[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (L::e[i][d] > 0) j[d] += f[i];
+      if (L::e[i][d] < 0) j[d] -= f[i];
+    }
+  }
+  const double rho = (double)ca.rho0 + drho;
+  if (!(rho >= 1e-10)) atomicOr(ca.status, 1);
+  const double inv = 1.0 / rho;
+  double u[3];
+#pragma unroll
+  for (int d = 0; d < D; d++) u[d] = j[d] * inv;
+  double uu = 0;
+#pragma unroll
+  for (int d = 0; d < D; d++) uu += u[d] * u[d];
+  const double c_uu = 1.5 * uu;
+  const double om = ca.omega64;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    double ue = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (L::e[i][d] > 0) ue += u[d];
+      if (L::e[i][d] < 0) ue -= u[d];
+    }
+    ue *= 3.0;
+    const double feq = L::w[i] * (drho + rho * (ue * (1.0 + 0.5 * ue) - c_uu));
+    f[i] -= om * (f[i] - feq);
+  }
+  if (write_macro && ca.rho) {
+    ca.rho[node] = (T)rho;
+#pragma unroll
+    for (int d = 0; d < D; d++) ca.u[d * ca.mstride + node] = (T)(u[d] * ca.scaling);
+  }
+}
+
+// FP64 collision of populations held in storage precision, storing each relaxed
+// population as soon as it is formed (f~ stays in T registers; only the moments
+// and one population at a time live in FP64).
+template <int Q, typename T>
+__device__ __forceinline__ void bgk_collide_store(const T (&f)[Q], const CollideArgs<T>& ca, int64_t node,
+                                                  bool write_macro, T* __restrict__ fout, int64_t stride) {
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  double drho = 0;
+#pragma unroll
+  for (int i = 0; i < Q; i++) drho += (double)f[i];
+  double j[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (L::e[i][d] > 0) j[d] += (double)f[i];
+      if (L::e[i][d] < 0) j[d] -= (double)f[i];
+    }
+  }
+  const double rho = (double)ca.rho0 + drho;
+  if (!(rho >= 1e-10)) atomicOr(ca.status, 1);
+  const double inv = 1.0 / rho;
+  double u[3];
+#pragma unroll
+  for (int d = 0; d < D; d++) u[d] = j[d] * inv;
+  double uu = 0;
+#pragma unroll
+  for (int d = 0; d < D; d++) uu += u[d] * u[d];
+  const double c_uu = 1.5 * uu;
+  const double om = ca.omega64;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    double ue = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (L::e[i][d] > 0) ue += u[d];
+      if (L::e[i][d] < 0) ue -= u[d];
+    }
+    ue *= 3.0;
+    const double feq = L::w[i] * (drho + rho * (ue * (1.0 + 0.5 * ue) - c_uu));
+    const double fi = (double)f[i];
+    fout[(int64_t)i * stride + node] = (T)(fi - om * (fi - feq));
+  }
+  if (write_macro && ca.rho) {
+    ca.rho[node] = (T)rho;
+#pragma unroll
+    for (int d = 0; d < D; d++) ca.u[d * ca.mstride + node] = (T)(u[d] * ca.scaling);
+  }
+}
+
 }  // namespace nat

@@ -15,7 +15,7 @@
 int nat_launch_assemble_general(nat_ctx* ctx, const double* d_disp, int* d_cell, double* d_xi);
 int nat_launch_assemble_axis(nat_ctx* ctx, const int* d_N, const double* d_vaxis, int maxc, const double* d_disp,
                              const int* d_nshift, double* d_w, int* d_off);
-int nat_launch_xi_to_f32(nat_ctx* ctx, const double* src, float* dst, int64_t n);
+int nat_launch_records_f32(nat_ctx* ctx, int64_t nrec);
 int nat_launch_sl_struct(nat_ctx* ctx, int write_macro);
 int nat_launch_sl_general(nat_ctx* ctx, int write_macro);
 int nat_launch_collide(nat_ctx* ctx);
@@ -397,15 +397,8 @@ int setup_structured_tables(nat_ctx* ctx, bool* ok) {
                 if (std::fabs(prod) < 1e-10) return 0;
               }
   }
-  // upload in compute precision
-  if (ctx->fp32) {
-    std::vector<float> wf(nw);
-    for (size_t i = 0; i < nw; i++) wf[i] = (float)w[i];
-    float* d;
-    TRY(alloc(ctx, &d, nw));
-    NAT_HIP(hipMemcpy(d, wf.data(), nw * sizeof(float), hipMemcpyHostToDevice));
-    ctx->ax_w = d;
-  } else {
+  // FP64 weights for both storage precisions (the kernel interpolates in FP64)
+  {
     double* d;
     TRY(alloc(ctx, &d, nw));
     NAT_HIP(hipMemcpy(d, w.data(), nw * sizeof(double), hipMemcpyHostToDevice));
@@ -437,7 +430,11 @@ int assemble_general(nat_ctx* ctx) {
   TRY(nat_launch_assemble_general(ctx, ddisp, ctx->dep_cell, ctx->dep_xi64));
   if (ctx->fp32) {
     if (!ctx->dep_xi32) TRY(alloc(ctx, &ctx->dep_xi32, nrec * ctx->D));
-    TRY(nat_launch_xi_to_f32(ctx, ctx->dep_xi64, ctx->dep_xi32, nrec * ctx->D));
+    if (!ctx->dep_code32) TRY(alloc(ctx, &ctx->dep_code32, nrec));
+    const int64_t nl = ctx->D == 3 ? (int64_t)(ctx->p + 1) * (ctx->p + 1) * (ctx->p + 1) : (int64_t)(ctx->p + 1) * (ctx->p + 1);
+    if ((int64_t)ctx->desc.ncells[0] * ctx->desc.ncells[1] * ctx->desc.ncells[2] * nl >= (int64_t)1 << 31)
+      return fail(ctx, NAT_ERR_UNSUPPORTED, "mesh too large for 32-bit FP32 departure records");
+    TRY(nat_launch_records_f32(ctx, nrec));
   }
   NAT_HIP(hipStreamSynchronize(ctx->stream));
   hipFree(ddisp);
@@ -465,7 +462,7 @@ void nat_destroy(nat_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   void* bufs[] = {ctx->f[0],     ctx->f[1],        ctx->g[0],     ctx->g[1],       ctx->rho,       ctx->u,
                   ctx->T,        ctx->mask,        ctx->d_vert,   ctx->d_gll,      ctx->d_coords,  ctx->d_owner,
-                  ctx->d_status, ctx->d_sums,      ctx->dep_cell, ctx->dep_xi64,   ctx->dep_xi32,  ctx->ax_w,
+                  ctx->d_status, ctx->d_sums,      ctx->dep_cell, ctx->dep_xi64,   ctx->dep_xi32,  ctx->dep_code32, ctx->ax_w,
                   ctx->ax_off,   ctx->sedg_df,     ctx->sedg_geo};
   for (void* b : bufs)
     if (b) hipFree(b);
@@ -496,6 +493,7 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
   ctx->Q = ctx->st.Q;
   ctx->p = d.order;
   ctx->fp32 = d.precision == NAT_FP32_SHIFTED;
+
   for (int a = 0; a < 3; a++) {
     if (a >= d.dim) {
       d.ncells[a] = 1;
@@ -696,7 +694,7 @@ int nat_collide(nat_ctx* ctx) {
   return check_status(ctx);
 }
 
-int nat_step(nat_ctx* ctx, int nsteps) {
+static int step_launch(nat_ctx* ctx, int nsteps) {
   if (!ctx->assembled) return fail(ctx, NAT_ERR_STATE, "operator not assembled");
   for (int s = 0; s < nsteps; s++) {
     const int wm = (s == nsteps - 1) ? 1 : 0;
@@ -710,7 +708,30 @@ int nat_step(nat_ctx* ctx, int nsteps) {
     }
     ctx->time += ctx->dt;
   }
+  return 0;
+}
+
+int nat_step(nat_ctx* ctx, int nsteps) {
+  TRY(step_launch(ctx, nsteps));
   return check_status(ctx);
+}
+
+int nat_step_timed(nat_ctx* ctx, int nsteps, double* elapsed_ms) {
+  hipEvent_t a, b;
+  NAT_HIP(hipEventCreate(&a));
+  NAT_HIP(hipEventCreate(&b));
+  NAT_HIP(hipEventRecord(a, ctx->stream));
+  int rc = step_launch(ctx, nsteps);
+  if (rc == 0) {
+    NAT_HIP(hipEventRecord(b, ctx->stream));
+    NAT_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    NAT_HIP(hipEventElapsedTime(&ms, a, b));
+    *elapsed_ms = ms;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return rc ? rc : check_status(ctx);
 }
 
 int nat_stream(nat_ctx* ctx) {

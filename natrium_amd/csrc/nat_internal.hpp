@@ -81,10 +81,15 @@ struct MeshDev {
   const double* vert;  // (N0+1)(N1+1)(N2+1) x 3 (z = 0 in 2D)
 };
 
+struct GllTab {
+  double x[MAXP + 1];
+};
+
 // Collision constants shared by all step kernels (unscaled lattice units).
 template <typename T>
 struct CollideArgs {
   T omega;     // 1/tau
+  double omega64;  // 1/tau for kernels that collide in FP64
   T rho0;      // FP32 shift reference density (0 for unshifted FP64)
   double scaling;
   T* rho;      // optional outputs (may be null)
@@ -133,7 +138,8 @@ struct nat_ctx {
   // SL general backend: departure records per (alpha-1, node)
   int* dep_cell = nullptr;
   double* dep_xi64 = nullptr;  // D per record
-  float* dep_xi32 = nullptr;
+  float* dep_xi32 = nullptr;   // FP32: offset from the anchor node, D per record
+  int* dep_code32 = nullptr;   // FP32: cell * (p+1)^D + anchor
 
   // SL structured backend: 1-D tables
   void* ax_w = nullptr;   // [3][MAXSHIFT][MAXP][MAXP+1] weights (T)
