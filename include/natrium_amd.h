@@ -67,6 +67,8 @@ typedef struct nat_desc {
   double gamma;          /* heat capacity ratio (compressible) */
   double rho0;           /* reference density of the FP32 shift f - w rho0 (0 -> 1.0) */
   int backend;           /* NAT_BACKEND_* (SL only) */
+  double prandtl;        /* compressible: Prandtl number (0 = not set, unit Prandtl) */
+  int sutherland;        /* compressible: 1 = Sutherland viscosity law (T/0.85)^0.7 */
 } nat_desc;
 
 /* Context lifetime. Replaces the CFDSolver<dim> constructor's set-up of the
@@ -118,14 +120,78 @@ int nat_step(nat_ctx* ctx, int nsteps);
  * and returns the elapsed device time in milliseconds. */
 int nat_step_timed(nat_ctx* ctx, int nsteps, double* elapsed_ms);
 
-/* macroscopic fields written by the last collision (FP64 host arrays; T may be NULL) */
+/* macroscopic fields written by the last collision (FP64 host arrays; T may be NULL,
+ * and must be NULL for incompressible contexts) */
 int nat_get_macro(nat_ctx* ctx, double* rho, double* u, double* T);
+
+/* Knudsen-number estimate sum_i |f_i - f_eq,i| / w_i / Q of the last compressible
+ * collision, the field behind the reference's shock mask (calculateKnudsenNumberEstimate,
+ * AuxiliaryCollisionFunctions.h:474-481; maskShockSensor, CollisionSchemes.h:90-100). */
+int nat_get_knudsen(nat_ctx* ctx, double* kn);
 
 /* PhysicalProperties::mass / kineticEnergy (PhysicalProperties.cpp:29-130, 270-310),
  * plus total momentum, reduced on the device in FP64 from the last collision's fields. */
 int nat_sums(nat_ctx* ctx, double* mass, double* momentum /* dim */, double* ekin);
 
 int nat_sync(nat_ctx* ctx);
+
+/* ------------------------------------------------------------------------
+ * Request server. No counterpart in the reference (one solve per process):
+ * continuous batching of independent solves on one GPU. Requests are admitted
+ * in arrival order while their device footprint fits the HBM budget and an
+ * active slot is free; every active request runs on its own stream in chunks of
+ * HIP-graph-captured fused steps, so requests overlap on the device and a
+ * retired request's memory admits the next one. A request's results do not
+ * depend on the other requests in flight. All functions are thread-safe.
+ * ------------------------------------------------------------------------ */
+enum { NAT_PROBLEM_TGV2D = 0, NAT_PROBLEM_TGV3D = 1, NAT_PROBLEM_SHOCK_TUBE = 2 };
+
+typedef struct nat_request {
+  int problem;     /* NAT_PROBLEM_*: TGV on [0,2pi]^D, or a density step (1 | 0.8) on a
+                      periodic 1 x 1/4 strip */
+  int stencil;     /* NAT_D2Q9 (2D problems), NAT_D3Q19 or NAT_D3Q27 (TGV3D) */
+  int refinement;  /* 2^refinement cells along x */
+  int order;       /* FE order p */
+  double reynolds; /* Re = U L / nu with the benchmark's length (2 pi in TGV2D, 1 otherwise) */
+  double mach;     /* Ma = U / c_s, U = 1 */
+  double cfl;      /* 0 -> 0.4 */
+  int nsteps;      /* fused stream+collide steps after the initial collision */
+  int precision;   /* NAT_FP64 or NAT_FP32_SHIFTED */
+} nat_request;
+
+typedef struct nat_result {
+  int status;              /* NAT_OK or the NAT_ERR_* the solve ended with */
+  int64_t n_dofs;
+  double queue_ms;         /* submit -> admitted */
+  double run_ms;           /* admitted -> retired (set-up, steps, read-back) */
+  double latency_ms;       /* submit -> retired */
+  double mass, ekin;       /* FP64 conservation sums after the last step */
+  int64_t footprint_bytes; /* device bytes reserved for the request */
+} nat_result;
+
+typedef struct nat_server_opts {
+  int64_t hbm_budget_bytes; /* <= 0: 80% of the free device memory at creation */
+  int max_active;           /* concurrently resident requests (<= 0: 8) */
+  int chunk_steps;          /* steps per scheduling chunk (<= 0: 50) */
+  int use_graphs;           /* 1: replay one captured HIP graph per chunk */
+} nat_server_opts;
+
+typedef struct nat_server_stats_t {
+  int64_t queued, active, used_bytes;
+  int64_t node_updates; /* DoF updates of completed requests */
+  double busy_ms;       /* first admission -> last retirement */
+} nat_server_stats_t;
+
+typedef struct nat_server nat_server;
+
+int nat_server_create(const nat_server_opts* opts, nat_server** out);
+int nat_server_submit(nat_server* s, const nat_request* r, int64_t* id);
+/* blocks until request id is retired; copies its velocity field (dim x n_dofs,
+ * physical units) into u when u_len > 0; the id is released afterwards */
+int nat_server_wait(nat_server* s, int64_t id, nat_result* res, double* u, int64_t u_len);
+int nat_server_stats(nat_server* s, nat_server_stats_t* st);
+/* finishes every queued request, then stops the scheduler */
+void nat_server_destroy(nat_server* s);
 
 #ifdef __cplusplus
 }

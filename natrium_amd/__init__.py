@@ -38,6 +38,7 @@ class _Desc(ctypes.Structure):
         ("viscosity", ctypes.c_double), ("cfl", ctypes.c_double), ("scheme", ctypes.c_int),
         ("precision", ctypes.c_int), ("equilibrium", ctypes.c_int), ("compressible", ctypes.c_int),
         ("gamma", ctypes.c_double), ("rho0", ctypes.c_double), ("backend", ctypes.c_int),
+        ("prandtl", ctypes.c_double), ("sutherland", ctypes.c_int),
     ]
 
 
@@ -45,7 +46,7 @@ EXPORTS = [
     "nat_create", "nat_destroy", "nat_last_error", "nat_n_dofs", "nat_n_stored_nodes", "nat_Q", "nat_dt",
     "nat_tau", "nat_time", "nat_backend", "nat_node_coords", "nat_set_dt", "nat_init_equilibrium",
     "nat_upload_f", "nat_download_f", "nat_upload_g", "nat_download_g", "nat_collide", "nat_stream",
-    "nat_step", "nat_step_timed", "nat_get_macro", "nat_sums", "nat_sync",
+    "nat_step", "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sync",
 ]
 
 _lib = None
@@ -91,6 +92,8 @@ def lib():
     L.nat_step_timed.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.nat_get_macro.restype = ctypes.c_int
     L.nat_get_macro.argtypes = [vp, _dp, _dp, _dp]
+    L.nat_get_knudsen.restype = ctypes.c_int
+    L.nat_get_knudsen.argtypes = [vp, _dp]
     L.nat_sums.restype = ctypes.c_int
     L.nat_sums.argtypes = [vp, _dp, _dp, _dp]
     _lib = L
@@ -110,10 +113,12 @@ class Solver:
 
     def __init__(self, stencil, order, ncells, vertices, cartesian=True, periodic=None, scaling=1.0,
                  viscosity=1.0, cfl=0.4, scheme="SL", precision="fp64", backend="auto", rho0=1.0,
-                 compressible=False, gamma=1.4, equilibrium="BGK"):
+                 compressible=False, gamma=1.4, equilibrium=None, prandtl=0.0, sutherland=False):
         L = lib()
         self.stencil = stencil
         self.D = 3 if stencil in ("D3Q19", "D3Q27") else 2
+        if equilibrium is None:
+            equilibrium = "QUARTIC" if compressible else "BGK"
         self.p = order
         self._verts = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, self.D)
         d = _Desc()
@@ -135,6 +140,8 @@ class Solver:
         d.gamma = gamma
         d.rho0 = rho0
         d.backend = BACKENDS[backend]
+        d.prandtl = prandtl
+        d.sutherland = int(sutherland)
         h = ctypes.c_void_p()
         rc = L.nat_create(ctypes.byref(d), ctypes.byref(h))
         self._h = h
@@ -239,11 +246,18 @@ class Solver:
         return ms.value
 
     def macro(self):
+        """rho, u (physical units) and T (None for incompressible contexts) of the last collision."""
         rho = np.zeros(self.n)
         u = np.zeros((self.D, self.n))
-        T = np.zeros(self.n)
-        self._check(lib().nat_get_macro(self._h, _ptr(rho), _ptr(u), _ptr(T)))
+        T = np.zeros(self.n) if self.compressible else None
+        self._check(lib().nat_get_macro(self._h, _ptr(rho), _ptr(u), _ptr(T) if T is not None else None))
         return rho, u, T
+
+    def knudsen(self):
+        """Knudsen-number estimate (shock sensor) of the last compressible collision."""
+        kn = np.zeros(self.n)
+        self._check(lib().nat_get_knudsen(self._h, _ptr(kn)))
+        return kn
 
     def sums(self):
         m = np.zeros(1)
@@ -254,3 +268,121 @@ class Solver:
 
     def sync(self):
         self._check(lib().nat_sync(self._h))
+
+
+# ----------------------------------------------------------------------------- request server
+PROBLEMS = {"TGV2D": 0, "TGV3D": 1, "SHOCK_TUBE": 2}
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [
+        ("problem", ctypes.c_int), ("stencil", ctypes.c_int), ("refinement", ctypes.c_int), ("order", ctypes.c_int),
+        ("reynolds", ctypes.c_double), ("mach", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("nsteps", ctypes.c_int), ("precision", ctypes.c_int),
+    ]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int), ("n_dofs", ctypes.c_int64), ("queue_ms", ctypes.c_double),
+        ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("mass", ctypes.c_double),
+        ("ekin", ctypes.c_double), ("footprint_bytes", ctypes.c_int64),
+    ]
+
+
+class _ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_active", ctypes.c_int), ("chunk_steps", ctypes.c_int),
+                ("use_graphs", ctypes.c_int)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("queued", ctypes.c_int64), ("active", ctypes.c_int64), ("used_bytes", ctypes.c_int64),
+                ("node_updates", ctypes.c_int64), ("busy_ms", ctypes.c_double)]
+
+
+SERVER_EXPORTS = ["nat_server_create", "nat_server_submit", "nat_server_wait", "nat_server_stats",
+                  "nat_server_destroy"]
+EXPORTS = EXPORTS + SERVER_EXPORTS
+
+
+def _server_lib():
+    L = lib()
+    if not getattr(L, "_server_bound", False):
+        vp = ctypes.c_void_p
+        L.nat_server_create.restype = ctypes.c_int
+        L.nat_server_create.argtypes = [ctypes.POINTER(_ServerOpts), ctypes.POINTER(vp)]
+        L.nat_server_submit.restype = ctypes.c_int
+        L.nat_server_submit.argtypes = [vp, ctypes.POINTER(_Request), ctypes.POINTER(ctypes.c_int64)]
+        L.nat_server_wait.restype = ctypes.c_int
+        L.nat_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(_Result), _dp, ctypes.c_int64]
+        L.nat_server_stats.restype = ctypes.c_int
+        L.nat_server_stats.argtypes = [vp, ctypes.POINTER(_ServerStats)]
+        L.nat_server_destroy.argtypes = [vp]
+        L._server_bound = True
+    return L
+
+
+class Server:
+    """Continuous-batching request server over the C ABI (one GPU).
+
+    submit() returns an id at once; wait(id) blocks until that solve is retired and
+    returns its timings, conservation sums and (optionally) its velocity field.
+    """
+
+    def __init__(self, hbm_budget_bytes=0, max_active=8, chunk_steps=50, use_graphs=True):
+        L = _server_lib()
+        o = _ServerOpts(int(hbm_budget_bytes), int(max_active), int(chunk_steps), int(bool(use_graphs)))
+        h = ctypes.c_void_p()
+        rc = L.nat_server_create(ctypes.byref(o), ctypes.byref(h))
+        if rc != 0:
+            raise NatError(rc, "server creation failed")
+        self._h = h
+
+    def submit(self, problem, stencil, refinement, order, reynolds, mach, nsteps, precision="fp64", cfl=0.4):
+        r = _Request(PROBLEMS[problem], STENCILS[stencil], int(refinement), int(order), float(reynolds), float(mach),
+                     float(cfl), int(nsteps), PRECISIONS[precision])
+        i = ctypes.c_int64()
+        rc = _server_lib().nat_server_submit(self._h, ctypes.byref(r), ctypes.byref(i))
+        if rc != 0:
+            raise NatError(rc, "request rejected")
+        return i.value
+
+    def wait(self, rid, want_u=False, n_dofs=None, dim=None):
+        res = _Result()
+        u = None
+        if want_u:
+            u = np.zeros(int(dim) * int(n_dofs))
+        rc = _server_lib().nat_server_wait(self._h, int(rid), ctypes.byref(res), _ptr(u) if u is not None else None,
+                                           0 if u is None else u.size)
+        out = {k: getattr(res, k) for k, _ in _Result._fields_}
+        out["status"] = rc
+        if u is not None:
+            out["u"] = u.reshape(int(dim), int(n_dofs))
+        return out
+
+    def stats(self):
+        st = _ServerStats()
+        _server_lib().nat_server_stats(self._h, ctypes.byref(st))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _server_lib().nat_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def request_dofs(problem, stencil, refinement, order):
+    """FE_Q DoF count and dimension of a server request's mesh."""
+    N = 2 ** refinement
+    if problem == "TGV3D":
+        return (N * order + 1) ** 3, 3
+    if problem == "TGV2D":
+        return (N * order + 1) ** 2, 2
+    M = max(N // 4, 1)
+    return (N * order + 1) * (M * order + 1), 2

@@ -1,0 +1,386 @@
+// Compressible populations (f, g) with the quartic Hermite equilibrium and FP64
+// temperature: SL streaming of both sets with one set of weights, fused with the
+// compressible BGK collision.
+//
+// Collision semantics (per node, FP64), following
+// collision_advanced/CollisionOperator.h:113-224, CollisionSchemes.h:43-118 and
+// AuxiliaryCollisionFunctions.h:290-307,421-431:
+//   rho = sum f, u = sum e f / rho (unscaled e)
+//   T   = 1/2 sum_i [ |e_i - u|^2 f_i / cs2 + g_i ] / (rho C_v),  C_v = 1/(gamma-1)
+//   f_eq = quartic Hermite polynomial (Equilibria.h:120-267), g_eq = f_eq T (2 C_v - D)
+//   tau_v = (tau - 1/2) S(T) / (T rho) + 1/2, S = (T/0.85)^0.7 with Sutherland's law
+//   optional Prandtl number: f* = w Q:(e e e - 3 cs2 e I)/(6 cs2^3), g* = w F.e / T
+//   f -= omega_v (f - f_eq) - (omega_v - omega_p) f*;  g likewise with g_eq, g*
+//   Knudsen estimate sum |f - f_eq| / w / Q written as the shock-sensor field.
+// The per-direction constants (unscaled e, w, |e|^2, the independent H3 and H4
+// components) come from a small FP64 table read with uniform addresses.
+//
+// FP32 storage keeps f~ = f - w rho0 and g~ = g - w rho0 (2 C_v - D), the
+// equilibria at rest with T = 1; the collision un-shifts in FP64.
+#include "nat_collide.hpp"
+
+namespace nat {
+
+constexpr int CTAB = 16;  // doubles per direction in the constant table
+enum { CT_E0 = 0, CT_E1 = 1, CT_W = 2, CT_EE = 3, CT_H3 = 4, CT_H4 = 8 };
+
+template <typename T>
+struct CompArgs {
+  const T* fin;
+  const T* gin;
+  T* fout;
+  T* gout;
+  int64_t stride;
+  int64_t n;
+  int nfe[3];
+  int N[3];
+  const int* cell;    // [(alpha-1) * n + i] departure cell (-1: identity row)
+  const double* xi;   // unit coordinates, 2 per record
+  double gll[MAXP + 1];
+  double inv_den[MAXP + 1];
+  double cs2;         // unscaled c_s^2
+  double tau;
+  double gamma;
+  double prandtl;
+  int prandtl_set;
+  int sutherland;
+  double rho0;        // FP32 shift of f (0 for FP64)
+  double gshift;      // FP32 shift factor of g: w rho0 (2 C_v - D)
+  double scaling;
+  T* rho;
+  T* u;
+  double* temp;
+  double* mask;
+  int64_t mstride;
+  int* status;
+};
+
+// Quartic equilibrium of direction q for the node scalars (2D).
+struct QuarticNode {
+  double rho, u0, u1, uu_term, Tm1, T1;
+  double a_xxx, a_xxy, a_xyy, a_yyy, a_xxxx, a_yyyy, a_xxxy, a_xyyy, a_xxyy;
+  __device__ void init(double r, double x, double y, double T, double cs2) {
+    rho = r;
+    u0 = x;
+    u1 = y;
+    uu_term = -(x * x) / (2.0 * cs2) - (y * y) / (2.0 * cs2);
+    Tm1 = T - 1;
+    T1 = cs2 * (T - 1);
+    a_xxx = x * x * x + T1 * (x + x + x);
+    a_xxy = x * x * y + T1 * y;
+    a_xyy = x * y * y + T1 * x;
+    a_yyy = y * y * y + T1 * (y + y + y);
+    a_xxxx = x * x * x * x + T1 * x * x * 6.0 + T1 * T1 * 3.0;
+    a_yyyy = y * y * y * y + T1 * y * y * 6.0 + T1 * T1 * 3.0;
+    a_xxxy = x * x * x * y + T1 * (x * y * 3.0);
+    a_xyyy = x * y * y * y + T1 * (x * y * 3.0);
+    a_xxyy = x * x * y * y + T1 * (x * x + y * y) + T1 * T1;
+  }
+  __device__ double feq(const double* __restrict__ t, double cs2) const {
+    const double e0 = t[CT_E0], e1 = t[CT_E1], w = t[CT_W];
+    const double ue = (u0 * e0) / cs2 + (u1 * e1) / cs2;
+    double f = w * rho * (1 + ue * (1 + 0.5 * ue) + uu_term);
+    f += rho * w / (2.0 * cs2) * (Tm1 * t[CT_EE] - 2.0 * cs2 * Tm1);
+    f += w * rho / (6. * cs2 * cs2 * cs2) *
+         (a_xxx * t[CT_H3 + 0] + 3 * (a_xxy * t[CT_H3 + 1] + a_xyy * t[CT_H3 + 2]) + a_yyy * t[CT_H3 + 3]);
+    f += w * rho / (24. * cs2 * cs2 * cs2 * cs2) *
+         (t[CT_H4 + 0] * a_xxxx + t[CT_H4 + 1] * a_yyyy + 6.0 * t[CT_H4 + 2] * a_xxyy + 4.0 * t[CT_H4 + 3] * a_xyyy +
+          4.0 * t[CT_H4 + 4] * a_xxxy);
+    return f;
+  }
+};
+
+// Compressible collision of one node; fl, gl unshifted FP64, relaxed in place.
+template <int Q, typename T>
+__device__ __forceinline__ void comp_collide(double (&fl)[Q], double (&gl)[Q], const CompArgs<T>& a,
+                                             const double* __restrict__ tab, int64_t node, bool write_macro) {
+  constexpr int D = 2;
+  const double cs2 = a.cs2;
+  const double C_v = 1. / (a.gamma - 1.0);
+  double rho = 0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) rho += fl[q];
+  if (!(rho >= 1e-10)) atomicOr(a.status, 1);
+  double j0 = 0, j1 = 0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    j0 += tab[q * CTAB + CT_E0] * fl[q];
+    j1 += tab[q * CTAB + CT_E1] * fl[q];
+  }
+  const double u0 = j0 * (1.0 / rho), u1 = j1 * (1.0 / rho);
+  double tsum = 0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double d0 = tab[q * CTAB + CT_E0] - u0, d1 = tab[q * CTAB + CT_E1] - u1;
+    tsum += (d0 * d0 + d1 * d1) * fl[q] / cs2 + gl[q];
+  }
+  const double temp = tsum * 0.5 / (rho * C_v);
+  QuarticNode qn;
+  qn.init(rho, u0, u1, temp, cs2);
+  const double gfac = temp * (2.0 * C_v - D);
+
+  double S = 1.0;
+  if (a.sutherland) S = pow(temp / 0.85, 0.7);
+  const double visc_tau = (a.tau - 0.5) * S / (temp * rho) + 0.5;
+  const double prandtl_tau = (visc_tau - 0.5) / (a.prandtl_set ? a.prandtl : 1.0) + 0.5;
+  const double om_v = 1. / visc_tau;
+  const double om_diff = om_v - 1. / prandtl_tau;
+
+  // Prandtl moments of the non-equilibrium parts
+  double Qt[2][2][2] = {{{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}};
+  double Fg[2] = {0, 0};
+  double kn = 0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double* t = tab + q * CTAB;
+    const double fe = qn.feq(t, cs2);
+    const double fneq = fl[q] - fe;
+    kn += fabs(fneq) / t[CT_W];
+    if (a.prandtl_set) {
+      const double gneq = gl[q] - fe * gfac;
+      const double d[2] = {t[CT_E0] - u0, t[CT_E1] - u1};
+      for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++)
+          for (int z = 0; z < 2; z++) Qt[x][y][z] += (d[x] * d[y] * d[z]) * fneq;
+      Fg[0] += d[0] * gneq;
+      Fg[1] += d[1] * gneq;
+    }
+  }
+  kn = kn / Q;
+  const double cs6 = 6.0 * cs2 * cs2 * cs2;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double* t = tab + q * CTAB;
+    const double fe = qn.feq(t, cs2);
+    const double ge = fe * gfac;
+    double fstar = 0, gstar = 0;
+    if (a.prandtl_set) {
+      const double e[2] = {t[CT_E0], t[CT_E1]};
+      for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++)
+          for (int z = 0; z < 2; z++)
+            fstar += t[CT_W] * (Qt[x][y][z] * (e[x] * e[y] * e[z] - 3 * cs2 * e[z] * (x == y ? 1.0 : 0.0))) / cs6;
+      gstar = t[CT_W] * (Fg[0] * e[0]) / temp + t[CT_W] * (Fg[1] * e[1]) / temp;
+    }
+    fl[q] -= om_v * (fl[q] - fe) - om_diff * fstar;
+    gl[q] -= om_v * (gl[q] - ge) - om_diff * gstar;
+  }
+  if (write_macro) {
+    a.rho[node] = (T)rho;
+    a.u[node] = (T)(u0 * a.scaling);
+    a.u[a.mstride + node] = (T)(u1 * a.scaling);
+    a.temp[node] = temp;
+    a.mask[node] = kn;
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void lag_all(const double (&x)[MAXP + 1], const double (&inv)[MAXP + 1], double xi,
+                                        double (&v)[P + 1]) {
+#pragma unroll
+  for (int j = 0; j <= P; j++) {
+    double s = inv[j];
+#pragma unroll
+    for (int m = 0; m <= P; m++)
+      if (m != j) s *= (xi - x[m]);
+    v[j] = s;
+  }
+}
+
+// fused SL stream (f and g, general backend records) + compressible collision
+template <int Q, int P, typename T>
+__global__ __launch_bounds__(256) void k_sl_comp(CompArgs<T> a, const double* __restrict__ tab, int write_macro) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double fl[Q], gl[Q];
+  fl[0] = (double)a.fin[i] + tab[CT_W] * a.rho0;
+  gl[0] = (double)a.gin[i] + tab[CT_W] * a.gshift;
+  const int n0 = a.nfe[0];
+#pragma unroll
+  for (int al = 1; al < Q; al++) {
+    const int64_t r = (int64_t)(al - 1) * a.n + i;
+    const T* fsrc = a.fin + (int64_t)al * a.stride;
+    const T* gsrc = a.gin + (int64_t)al * a.stride;
+    const int c = a.cell[r];
+    double af, ag;
+    if (c < 0) {
+      af = (double)fsrc[i];
+      ag = (double)gsrc[i];
+    } else {
+      double lx[P + 1], ly[P + 1];
+      lag_all<P>(a.gll, a.inv_den, a.xi[2 * r + 0], lx);
+      lag_all<P>(a.gll, a.inv_den, a.xi[2 * r + 1], ly);
+      const int cx = c % a.N[0], cy = c / a.N[0];
+      const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P);
+      af = 0;
+      ag = 0;
+#pragma unroll
+      for (int my = 0; my <= P; my++)
+#pragma unroll
+        for (int mx = 0; mx <= P; mx++) {
+          const double w = lx[mx] * ly[my];
+          if (!(fabs(w) < 1e-10)) {
+            const int64_t k = base + mx + (int64_t)n0 * my;
+            af += w * (double)fsrc[k];
+            ag += w * (double)gsrc[k];
+          }
+        }
+    }
+    // shifted storage: interpolation preserves constants, so the shift is added back here
+    fl[al] = af + tab[al * CTAB + CT_W] * a.rho0;
+    gl[al] = ag + tab[al * CTAB + CT_W] * a.gshift;
+  }
+  comp_collide<Q, T>(fl, gl, a, tab, i, write_macro != 0);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    a.fout[(int64_t)q * a.stride + i] = (T)(fl[q] - tab[q * CTAB + CT_W] * a.rho0);
+    a.gout[(int64_t)q * a.stride + i] = (T)(gl[q] - tab[q * CTAB + CT_W] * a.gshift);
+  }
+}
+
+// collision only, in place
+template <int Q, typename T>
+__global__ __launch_bounds__(256) void k_collide_comp(CompArgs<T> a, const double* __restrict__ tab) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double fl[Q], gl[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    fl[q] = (double)a.fin[(int64_t)q * a.stride + i] + tab[q * CTAB + CT_W] * a.rho0;
+    gl[q] = (double)a.gin[(int64_t)q * a.stride + i] + tab[q * CTAB + CT_W] * a.gshift;
+  }
+  comp_collide<Q, T>(fl, gl, a, tab, i, true);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    a.fout[(int64_t)q * a.stride + i] = (T)(fl[q] - tab[q * CTAB + CT_W] * a.rho0);
+    a.gout[(int64_t)q * a.stride + i] = (T)(gl[q] - tab[q * CTAB + CT_W] * a.gshift);
+  }
+}
+
+template <typename T>
+static void fill_comp(nat_ctx* ctx, CompArgs<T>& a, int src, int dst) {
+  a.fin = (const T*)ctx->f[src];
+  a.gin = (const T*)ctx->g[src];
+  a.fout = (T*)ctx->f[dst];
+  a.gout = (T*)ctx->g[dst];
+  a.stride = ctx->stride;
+  a.n = ctx->n_dofs;
+  for (int d = 0; d < 3; d++) {
+    a.nfe[d] = ctx->nfe[d];
+    a.N[d] = d < ctx->D ? ctx->desc.ncells[d] : 1;
+  }
+  a.cell = ctx->dep_cell;
+  a.xi = ctx->dep_xi64;
+  double x[MAXP + 1], w[MAXP + 1];
+  gll_points(ctx->p, x, w);
+  for (int j = 0; j <= MAXP; j++) {
+    a.gll[j] = j <= ctx->p ? x[j] : 0.0;
+    double d = 1;
+    for (int m = 0; m <= ctx->p; m++)
+      if (m != j && j <= ctx->p) d *= (x[j] - x[m]);
+    a.inv_den[j] = j <= ctx->p ? 1.0 / d : 0.0;
+  }
+  const double s = ctx->st.s;
+  a.cs2 = ctx->st.cs2 / (s * s);
+  a.tau = ctx->tau;
+  a.gamma = ctx->desc.gamma;
+  a.prandtl = ctx->desc.prandtl;
+  a.prandtl_set = ctx->desc.prandtl > 0 ? 1 : 0;
+  a.sutherland = ctx->desc.sutherland;
+  const double C_v = 1. / (ctx->desc.gamma - 1.0);
+  const double rho0 = ctx->fp32 ? (ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0) : 0.0;
+  a.rho0 = rho0;
+  a.gshift = rho0 * (2.0 * C_v - ctx->D);
+  a.scaling = s;
+  a.rho = (T*)ctx->rho;
+  a.u = (T*)ctx->u;
+  a.temp = ctx->T;
+  a.mask = ctx->mask;
+  a.mstride = ctx->n_store;
+  a.status = ctx->d_status;
+}
+
+template <typename T>
+static int step_comp(nat_ctx* ctx, int write_macro) {
+  CompArgs<T> a;
+  const int src = ctx->cur, dst = 1 - ctx->cur;
+  fill_comp<T>(ctx, a, src, dst);
+  const unsigned grid = (unsigned)((a.n + 255) / 256);
+  const double* tab = ctx->comp_tab;
+  switch (ctx->p) {
+    case 1: hipLaunchKernelGGL((k_sl_comp<25, 1, T>), dim3(grid), dim3(256), 0, ctx->stream, a, tab, write_macro); break;
+    case 2: hipLaunchKernelGGL((k_sl_comp<25, 2, T>), dim3(grid), dim3(256), 0, ctx->stream, a, tab, write_macro); break;
+    case 3: hipLaunchKernelGGL((k_sl_comp<25, 3, T>), dim3(grid), dim3(256), 0, ctx->stream, a, tab, write_macro); break;
+    case 4: hipLaunchKernelGGL((k_sl_comp<25, 4, T>), dim3(grid), dim3(256), 0, ctx->stream, a, tab, write_macro); break;
+    default:
+      ctx->err = "compressible SL kernel instantiated for p <= 4";
+      return NAT_ERR_UNSUPPORTED;
+  }
+  NAT_HIP(hipGetLastError());
+  ctx->cur = dst;
+  return 0;
+}
+
+template <typename T>
+static int collide_comp(nat_ctx* ctx) {
+  CompArgs<T> a;
+  fill_comp<T>(ctx, a, ctx->cur, ctx->cur);
+  const unsigned grid = (unsigned)((a.n + 255) / 256);
+  hipLaunchKernelGGL((k_collide_comp<25, T>), dim3(grid), dim3(256), 0, ctx->stream, a, (const double*)ctx->comp_tab);
+  NAT_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace nat
+
+// Constant table: unscaled e, w, |e|^2 and the independent H3 / H4 components
+// (AuxiliaryCollisionFunctions.h:519-566) per direction.
+int nat_comp_setup(nat_ctx* ctx) {
+  using namespace nat;
+  if (ctx->D != 2 || ctx->Q != 25) {
+    ctx->err = "compressible populations are implemented for the D2Q25H stencil";
+    return NAT_ERR_UNSUPPORTED;
+  }
+  const double s = ctx->st.s;
+  const double cs2 = ctx->st.cs2 / (s * s);
+  std::vector<double> t((size_t)ctx->Q * CTAB, 0.0);
+  for (int q = 0; q < ctx->Q; q++) {
+    const double e[2] = {ctx->st.dir[q][0] / s, ctx->st.dir[q][1] / s};
+    double* r = &t[(size_t)q * CTAB];
+    r[CT_E0] = e[0];
+    r[CT_E1] = e[1];
+    r[CT_W] = ctx->st.w[q];
+    r[CT_EE] = e[0] * e[0] + e[1] * e[1];
+    auto eye = [](int a, int b) { return a == b ? 1.0 : 0.0; };
+    auto H3 = [&](int a, int b, int c) {
+      return e[a] * e[b] * e[c] - cs2 * (e[a] * eye(b, c) + e[b] * eye(a, c) + e[c] * eye(a, b));
+    };
+    auto H4 = [&](int a, int b, int c, int d) {
+      const double p4 = e[a] * e[b] * e[c] * e[d];
+      const double p2 = e[a] * e[b] * eye(c, d) + e[a] * e[c] * eye(b, d) + e[a] * e[d] * eye(b, c) +
+                        e[b] * e[c] * eye(a, d) + e[b] * e[d] * eye(a, c) + e[c] * e[d] * eye(a, b);
+      const double p0 = eye(a, b) * eye(c, d) + eye(a, c) * eye(b, d) + eye(a, d) * eye(b, c);
+      return p4 - cs2 * p2 + cs2 * cs2 * p0;
+    };
+    r[CT_H3 + 0] = H3(0, 0, 0);
+    r[CT_H3 + 1] = H3(0, 0, 1);
+    r[CT_H3 + 2] = H3(0, 1, 1);
+    r[CT_H3 + 3] = H3(1, 1, 1);
+    r[CT_H4 + 0] = H4(0, 0, 0, 0);
+    r[CT_H4 + 1] = H4(1, 1, 1, 1);
+    r[CT_H4 + 2] = H4(0, 0, 1, 1);
+    r[CT_H4 + 3] = H4(0, 1, 1, 1);
+    r[CT_H4 + 4] = H4(0, 0, 0, 1);
+  }
+  if (!ctx->comp_tab) NAT_HIP(hipMalloc(&ctx->comp_tab, t.size() * sizeof(double)));
+  NAT_HIP(hipMemcpy(ctx->comp_tab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int nat_launch_sl_comp(nat_ctx* ctx, int write_macro) {
+  return ctx->fp32 ? nat::step_comp<float>(ctx, write_macro) : nat::step_comp<double>(ctx, write_macro);
+}
+
+int nat_launch_collide_comp(nat_ctx* ctx) {
+  return ctx->fp32 ? nat::collide_comp<float>(ctx) : nat::collide_comp<double>(ctx);
+}

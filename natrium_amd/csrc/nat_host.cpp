@@ -22,6 +22,9 @@ int nat_launch_collide(nat_ctx* ctx);
 int nat_launch_sums(nat_ctx* ctx, int layout);
 int nat_sedg_setup(nat_ctx* ctx);
 int nat_sedg_step(nat_ctx* ctx, int collide_after, int write_macro);
+int nat_comp_setup(nat_ctx* ctx);
+int nat_launch_sl_comp(nat_ctx* ctx, int write_macro);
+int nat_launch_collide_comp(nat_ctx* ctx);
 
 namespace nat {
 
@@ -463,7 +466,7 @@ void nat_destroy(nat_ctx* ctx) {
   void* bufs[] = {ctx->f[0],     ctx->f[1],        ctx->g[0],     ctx->g[1],       ctx->rho,       ctx->u,
                   ctx->T,        ctx->mask,        ctx->d_vert,   ctx->d_gll,      ctx->d_coords,  ctx->d_owner,
                   ctx->d_status, ctx->d_sums,      ctx->dep_cell, ctx->dep_xi64,   ctx->dep_xi32,  ctx->dep_code32, ctx->ax_w,
-                  ctx->ax_off,   ctx->sedg_df,     ctx->sedg_geo};
+                  ctx->ax_off,   ctx->sedg_df,     ctx->sedg_geo, ctx->comp_tab};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -488,7 +491,16 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
   if (d.scheme != NAT_SL && d.scheme != NAT_SEDG && d.scheme != NAT_SEDG_CENTRAL) return fail(ctx, NAT_ERR_ARG, "unknown scheme");
   if (d.equilibrium != NAT_EQ_BGK && !d.compressible)
     return fail(ctx, NAT_ERR_UNSUPPORTED, "incompressible collision supports the BGK equilibrium only");
-  if (d.compressible) return fail(ctx, NAT_ERR_UNSUPPORTED, "compressible populations are not enabled in this build");
+  if (d.compressible) {
+    // selectCollision: compressible (2, 25, BGK_STANDARD, QUARTIC_EQUILIBRIUM) (CollisionSelection.h:143)
+    if (d.stencil != NAT_D2Q25H) return fail(ctx, NAT_ERR_UNSUPPORTED, "compressible populations need the D2Q25H stencil");
+    if (d.scheme != NAT_SL) return fail(ctx, NAT_ERR_UNSUPPORTED, "compressible populations use SL streaming");
+    if (!(d.gamma > 1.0)) return fail(ctx, NAT_ERR_ARG, "gamma must be > 1");
+    if (d.prandtl < 0) return fail(ctx, NAT_ERR_ARG, "prandtl must be >= 0");
+    d.backend = NAT_BACKEND_GENERAL;
+  } else if (d.stencil == NAT_D2Q25H) {
+    return fail(ctx, NAT_ERR_UNSUPPORTED, "the D2Q25H stencil is used with compressible populations only");
+  }
   ctx->D = d.dim;
   ctx->Q = ctx->st.Q;
   ctx->p = d.order;
@@ -568,6 +580,18 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
     if (e != hipSuccess) return fail(ctx, NAT_ERR_NOMEM, "device allocation of populations failed");
     NAT_HIP(hipMemset(ctx->f[b], 0, es * ctx->stride * ctx->Q));
   }
+  if (d.compressible) {
+    for (int b = 0; b < 2; b++) {
+      hipError_t e = hipMalloc(&ctx->g[b], es * ctx->stride * ctx->Q);
+      if (e != hipSuccess) return fail(ctx, NAT_ERR_NOMEM, "device allocation of g populations failed");
+      NAT_HIP(hipMemset(ctx->g[b], 0, es * ctx->stride * ctx->Q));
+    }
+    TRY(alloc(ctx, &ctx->T, ctx->n_store));
+    TRY(alloc(ctx, &ctx->mask, ctx->n_store));
+    NAT_HIP(hipMemset(ctx->T, 0, sizeof(double) * ctx->n_store));
+    NAT_HIP(hipMemset(ctx->mask, 0, sizeof(double) * ctx->n_store));
+    TRY(nat_comp_setup(ctx));
+  }
   {
     hipError_t e = hipMalloc(&ctx->rho, es * ctx->n_store);
     if (e == hipSuccess) e = hipMalloc(&ctx->u, es * ctx->n_store * ctx->D);
@@ -615,11 +639,17 @@ int nat_set_dt(nat_ctx* ctx, double dt) {
   return assemble_general(ctx);
 }
 
-static int upload_pop(nat_ctx* ctx, void* dst, const double* f) {
+// FP32 shift of the g populations per unit weight: rho0 (2 C_v - D), the g equilibrium at rest with T = 1
+static double g_shift(const nat_ctx* ctx) {
+  const double rho0 = ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0;
+  return rho0 * (2.0 / (ctx->desc.gamma - 1.0) - ctx->D);
+}
+
+static int upload_pop(nat_ctx* ctx, void* dst, const double* f, bool is_g = false) {
   const int Q = ctx->Q;
   const int64_t n = ctx->n_dofs;
   if (ctx->fp32) {
-    const double rho0 = ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0;
+    const double rho0 = is_g ? g_shift(ctx) : (ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0);
     std::vector<float> h((size_t)ctx->stride * Q, 0.0f);
     for (int q = 0; q < Q; q++)
       for (int64_t i = 0; i < n; i++) h[(size_t)q * ctx->stride + store_index(ctx, i)] = (float)(f[q * n + i] - ctx->st.w[q] * rho0);
@@ -633,12 +663,12 @@ static int upload_pop(nat_ctx* ctx, void* dst, const double* f) {
   return 0;
 }
 
-static int download_pop(nat_ctx* ctx, const void* src, double* f) {
+static int download_pop(nat_ctx* ctx, const void* src, double* f, bool is_g = false) {
   const int Q = ctx->Q;
   const int64_t n = ctx->n_dofs;
   NAT_HIP(hipStreamSynchronize(ctx->stream));
   if (ctx->fp32) {
-    const double rho0 = ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0;
+    const double rho0 = is_g ? g_shift(ctx) : (ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0);
     std::vector<float> h((size_t)ctx->stride * Q);
     NAT_HIP(hipMemcpy(h.data(), src, h.size() * sizeof(float), hipMemcpyDeviceToHost));
     for (int q = 0; q < Q; q++)
@@ -657,11 +687,76 @@ int nat_upload_f(nat_ctx* ctx, const double* f) {
   return upload_pop(ctx, ctx->f[ctx->cur], f);
 }
 int nat_download_f(nat_ctx* ctx, double* f) { return download_pop(ctx, ctx->f[ctx->cur], f); }
-int nat_upload_g(nat_ctx* ctx, const double*) { return fail(ctx, NAT_ERR_UNSUPPORTED, "no g populations in this context"); }
-int nat_download_g(nat_ctx* ctx, double*) { return fail(ctx, NAT_ERR_UNSUPPORTED, "no g populations in this context"); }
+int nat_upload_g(nat_ctx* ctx, const double* g) {
+  if (!ctx->desc.compressible) return fail(ctx, NAT_ERR_UNSUPPORTED, "no g populations in this context");
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  return upload_pop(ctx, ctx->g[ctx->cur], g, true);
+}
+int nat_download_g(nat_ctx* ctx, double* g) {
+  if (!ctx->desc.compressible) return fail(ctx, NAT_ERR_UNSUPPORTED, "no g populations in this context");
+  return download_pop(ctx, ctx->g[ctx->cur], g, true);
+}
+
+// CompressibleCFDSolver::calcQuarticEquilibrium in its loop form
+// (CompressibleCFDSolver.h:790-848), g = f_eq T (2 C_v - D) (:930-943)
+static int init_compressible(nat_ctx* ctx, const double* rho, const double* u, const double* T) {
+  const int Q = ctx->Q, D = ctx->D;
+  const int64_t n = ctx->n_dofs;
+  const double s = ctx->st.s;
+  const double cs2 = ctx->st.cs2 / (s * s);
+  const double C_v = 1. / (ctx->desc.gamma - 1.0);
+  std::vector<double> f((size_t)Q * n), g((size_t)Q * n);
+  auto eye = [](int a, int b) { return a == b ? 1.0 : 0.0; };
+  for (int64_t ii = 0; ii < n; ii++) {
+    double vel[3] = {0, 0, 0};
+    for (int j = 0; j < D; j++) vel[j] = u[j * n + ii] / s;
+    const double density = rho[ii], temperature = T[ii];
+    double uu_term = 0.0;
+    for (int j = 0; j < D; j++) uu_term += -(vel[j] * vel[j]) / (2.0 * cs2);
+    for (int i = 0; i < Q; i++) {
+      double e[3] = {0, 0, 0};
+      for (int j = 0; j < D; j++) e[j] = ctx->st.dir[i][j] / s;
+      const double w = ctx->st.w[i];
+      const double T1 = cs2 * (temperature - 1);
+      double ue = 0.0;
+      for (int j = 0; j < D; j++) ue += (vel[j] * e[j]) / cs2;
+      double feq = w * density * (1 + ue * (1 + 0.5 * ue) + uu_term);
+      for (int a = 0; a < D; a++)
+        for (int b = 0; b < D; b++) {
+          feq += density * w / (2.0 * cs2) *
+                 ((temperature - 1) * eye(a, b) * e[a] * e[b] - cs2 * eye(a, b) * (temperature - 1));
+          for (int c = 0; c < D; c++) {
+            feq += w * density / (6. * cs2 * cs2 * cs2) *
+                   (vel[a] * vel[b] * vel[c] + T1 * (eye(a, b) * vel[c] + eye(b, c) * vel[a] + eye(a, c) * vel[b])) *
+                   (e[a] * e[b] * e[c] - cs2 * (e[c] * eye(a, b) + e[b] * eye(a, c) + e[a] * eye(b, c)));
+            for (int d = 0; d < D; d++) {
+              const double p4 = e[a] * e[b] * e[c] * e[d];
+              const double p2 = e[a] * e[b] * eye(c, d) + e[a] * e[c] * eye(b, d) + e[a] * e[d] * eye(b, c) +
+                                e[b] * e[c] * eye(a, d) + e[b] * e[d] * eye(a, c) + e[c] * e[d] * eye(a, b);
+              const double p0 = eye(a, b) * eye(c, d) + eye(a, c) * eye(b, d) + eye(a, d) * eye(b, c);
+              const double u4 = vel[a] * vel[b] * vel[c] * vel[d];
+              const double u2 = vel[a] * vel[b] * eye(c, d) + vel[a] * vel[c] * eye(b, d) + vel[a] * vel[d] * eye(b, c) +
+                                vel[b] * vel[c] * eye(a, d) + vel[b] * vel[d] * eye(a, c) + vel[c] * vel[d] * eye(a, b);
+              feq += w * density / (24. * cs2 * cs2 * cs2 * cs2) * (p4 - cs2 * p2 + cs2 * cs2 * p0) *
+                     (u4 + T1 * (u2 + T1 * p0));
+            }
+          }
+        }
+      f[(size_t)i * n + ii] = feq;
+      g[(size_t)i * n + ii] = feq * temperature * (2.0 * C_v - D);
+    }
+  }
+  ctx->time = 0;
+  ctx->collided = false;
+  TRY(nat_upload_f(ctx, f.data()));
+  return nat_upload_g(ctx, g.data());
+}
 
 int nat_init_equilibrium(nat_ctx* ctx, const double* rho, const double* u, const double* T) {
-  (void)T;
+  if (ctx->desc.compressible) {
+    if (!T) return fail(ctx, NAT_ERR_ARG, "compressible initialisation needs the temperature field");
+    return init_compressible(ctx, rho, u, T);
+  }
   // BGKStandard::getEquilibriumDistribution with the scaled stencil (BGKStandard.cpp:23-41)
   const int Q = ctx->Q, D = ctx->D;
   const int64_t n = ctx->n_dofs;
@@ -689,7 +784,10 @@ int nat_init_equilibrium(nat_ctx* ctx, const double* rho, const double* u, const
 }
 
 int nat_collide(nat_ctx* ctx) {
-  TRY(nat_launch_collide(ctx));
+  if (ctx->desc.compressible)
+    TRY(nat_launch_collide_comp(ctx));
+  else
+    TRY(nat_launch_collide(ctx));
   ctx->collided = true;
   return check_status(ctx);
 }
@@ -698,7 +796,9 @@ static int step_launch(nat_ctx* ctx, int nsteps) {
   if (!ctx->assembled) return fail(ctx, NAT_ERR_STATE, "operator not assembled");
   for (int s = 0; s < nsteps; s++) {
     const int wm = (s == nsteps - 1) ? 1 : 0;
-    if (ctx->desc.scheme == NAT_SL) {
+    if (ctx->desc.compressible) {
+      TRY(nat_launch_sl_comp(ctx, wm));
+    } else if (ctx->desc.scheme == NAT_SL) {
       if (ctx->backend == NAT_BACKEND_STRUCTURED)
         TRY(nat_launch_sl_struct(ctx, wm));
       else
@@ -741,11 +841,21 @@ int nat_stream(nat_ctx* ctx) {
   return check_status(ctx);
 }
 
+int nat_get_knudsen(nat_ctx* ctx, double* kn) {
+  if (!ctx->desc.compressible) return fail(ctx, NAT_ERR_UNSUPPORTED, "no shock sensor in an incompressible context");
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  NAT_HIP(hipMemcpy(kn, ctx->mask, ctx->n_store * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int nat_get_macro(nat_ctx* ctx, double* rho, double* u, double* T) {
-  (void)T;
   NAT_HIP(hipStreamSynchronize(ctx->stream));
   const int64_t n = ctx->n_dofs, ns = ctx->n_store;
   const int D = ctx->D;
+  if (T) {
+    if (!ctx->desc.compressible) return fail(ctx, NAT_ERR_UNSUPPORTED, "no temperature in an incompressible context");
+    NAT_HIP(hipMemcpy(T, ctx->T, ns * sizeof(double), hipMemcpyDeviceToHost));
+  }
   if (ctx->fp32) {
     std::vector<float> r(ns), v((size_t)ns * D);
     NAT_HIP(hipMemcpy(r.data(), ctx->rho, ns * sizeof(float), hipMemcpyDeviceToHost));
@@ -787,3 +897,43 @@ int nat_sync(nat_ctx* ctx) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Internal entry points for the request server (nat_serve.cpp); not exported.
+// ---------------------------------------------------------------------------
+int nat_internal_enqueue(nat_ctx* ctx, int nsteps) { return step_launch(ctx, nsteps); }
+int nat_internal_status(nat_ctx* ctx) { return check_status(ctx); }
+
+// Device bytes a context for this descriptor allocates (populations, macroscopic
+// fields, departure records or SEDG work vectors; tables are ignored).
+int64_t nat_internal_footprint(const nat_desc* d) {
+  using namespace nat;
+  const int D = d->dim, p = d->order;
+  const int Q = d->stencil == NAT_D2Q9 ? 9 : d->stencil == NAT_D3Q19 ? 19 : d->stencil == NAT_D3Q27 ? 27 : 25;
+  const int np = p + 1;
+  int64_t ncell = 1, nfe = 1, nu = 1;
+  for (int a = 0; a < D; a++) {
+    ncell *= d->ncells[a];
+    nfe *= (int64_t)d->ncells[a] * p + 1;
+    nu *= (int64_t)d->ncells[a] * p;
+  }
+  const int64_t es = d->precision == NAT_FP32_SHIFTED ? 4 : 8;
+  int64_t n_store, recs = 0;
+  if (d->scheme != NAT_SL) {
+    n_store = ncell * (D == 3 ? np * np * np : np * np);
+    recs = Q * ((n_store + 127) / 128 * 128) * es;  // RK5 df
+  } else {
+    const bool structured = !d->compressible && d->cartesian && d->backend != NAT_BACKEND_GENERAL &&
+                            structured_instantiated(D, Q, p);
+    n_store = structured ? nu : nfe;
+    if (!structured) {
+      recs = (int64_t)(Q - 1) * nfe * (4 + 8 * D);
+      if (es == 4 && !d->compressible) recs += (int64_t)(Q - 1) * nfe * (4 + 4 * D);
+    }
+  }
+  const int64_t stride = (n_store + 127) / 128 * 128;
+  int64_t bytes = 2 * Q * stride * es * (d->compressible ? 2 : 1);
+  bytes += (1 + D) * n_store * es + (d->compressible ? 2 * 8 * n_store : 0);
+  bytes += recs;
+  return bytes + (1 << 20);
+}

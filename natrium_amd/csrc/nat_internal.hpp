@@ -152,6 +152,9 @@ struct nat_ctx {
   void* sedg_df = nullptr;
   double* sedg_geo = nullptr;
 
+  // compressible: per-direction constant table (unscaled e, w, H3/H4 components)
+  double* comp_tab = nullptr;
+
   std::string err;
 };
 

@@ -67,6 +67,7 @@ def lib():
         L.orc_quartic_feq.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_double, _dp]
         L.orc_rk5_scalar.restype = ctypes.c_double
         L.orc_rk5_scalar.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_set_compressible_options.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -127,7 +128,7 @@ class Oracle:
     """CFDSolver-like FP64 CPU solver (SL or SEDG, incompressible or compressible)."""
 
     def __init__(self, stencil, p, ncells, vertices, cartesian=True, periodic=None, scaling=1.0, viscosity=1.0,
-                 cfl=0.4, scheme="SL", compressible=False, gamma=1.4):
+                 cfl=0.4, scheme="SL", compressible=False, gamma=1.4, prandtl=0.0, sutherland=False):
         sid, D, Q = STENCILS[stencil]
         self.D, self.Q, self.p = D, Q, p
         nc = np.array(list(ncells) + [1] * (3 - len(ncells)), dtype=np.int32)
@@ -140,6 +141,8 @@ class Oracle:
             raise RuntimeError(err)
         self.n = lib().orc_n_dofs(self.h)
         self.compressible = compressible
+        if compressible:
+            lib().orc_set_compressible_options(self.h, float(prandtl), int(sutherland))
 
     def __del__(self):
         if getattr(self, "h", None):

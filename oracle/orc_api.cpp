@@ -97,6 +97,14 @@ orc_solver* orc_create(int dim, int stencil_id, int p, const int* ncells, const 
   return s;
 }
 
+// Compressible options of the reference configuration (SolverConfiguration
+// "Prandtl number" and "Sutherland law"): prandtl <= 0 leaves it unset.
+void orc_set_compressible_options(orc_solver* s, double prandtl, int sutherland) {
+  s->cp.prandtl_set = prandtl > 0;
+  s->cp.prandtl = prandtl > 0 ? prandtl : 1.0;
+  s->cp.sutherland_set = sutherland != 0;
+}
+
 const char* orc_error(orc_solver* s) { return s->err.c_str(); }
 void orc_destroy(orc_solver* s) { delete s; }
 int64_t orc_n_dofs(orc_solver* s) { return s->n; }

@@ -470,3 +470,33 @@ def test_tgv2d_sl_decay_reference_bands():
     _, uu, _, _ = o.macro()
     _, uref = tgv2d_fields(x, nu, cs, L, t=o.time)
     assert np.sqrt(((uu - uref) ** 2).sum() / (uref ** 2).sum()) < 5e-2
+
+
+def test_compressible_prandtl_and_sutherland_paths():
+    """Unit Prandtl number equals the unset option; mass and momentum stay conserved with Pr != 1."""
+    s = 1.7
+    N = [2, 2]
+    v = box_vertices(2, N, [0, 0], [1, 1])
+    rng = np.random.default_rng(5)
+    outs = []
+    for pr, su in [(0.0, False), (1.0, False), (0.71, True)]:
+        o = O.Oracle("D2Q25H", 1, N, v, scaling=s, viscosity=0.02, compressible=True, gamma=1.4, prandtl=pr,
+                     sutherland=su)
+        if not outs:
+            rho = 1 + 0.2 * rng.random(o.n)
+            u = 0.2 * s * (rng.random((2, o.n)) - 0.5)
+            T = 0.8 + 0.4 * rng.random(o.n)
+            o.init(rho, u, T)
+            f = o.get_f() * (1 + 0.02 * rng.standard_normal((25, o.n)))
+            g = o.get_g() * (1 + 0.02 * rng.standard_normal((25, o.n)))
+        o.set_f(f)
+        o.set_g(g)
+        o.collide()
+        outs.append((o.get_f(), o.get_g()))
+    np.testing.assert_allclose(outs[1][0], outs[0][0], rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(outs[1][1], outs[0][1], rtol=1e-13, atol=1e-15)
+    e, w, opp, cs2 = O.stencil("D2Q25H", s)
+    f2 = outs[2][0]
+    assert np.abs(f2 - outs[0][0]).max() > 1e-8  # the Prandtl/Sutherland path changes the result
+    np.testing.assert_allclose(f2.sum(0), f.sum(0), rtol=1e-13)
+    np.testing.assert_allclose((e / s).T @ f2, (e / s).T @ f, rtol=0, atol=1e-13)
