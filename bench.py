@@ -97,6 +97,83 @@ def cpu_baseline(prob, seconds):
                       f"{o.n} FE_Q DoFs, {el:.1f} s wall, OpenMP {threads} threads; assembly untimed"}
 
 
+def serving_requests(n, seed, full):
+    """Seeded serving mix: 2D/3D Taylor-Green or a shock tube on a 2D strip, refinement 3-5,
+    p in {2,3,4,6}, Re log-uniform [10,1600], Ma U[0.05,0.2], steps log-uniform [100,5000].
+    The default (not full) run caps refinement at 4 and steps at 1000 to stay within a minute."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        prob = ["TGV2D", "TGV3D", "SHOCK_TUBE"][rng.integers(3)]
+        ref = int(rng.integers(3, 6 if full else 5))
+        p = int([2, 3, 4, 6][rng.integers(4)])
+        st = "D2Q9" if prob != "TGV3D" else ["D3Q19", "D3Q27"][rng.integers(2)]
+        if prob == "TGV3D" and not full:
+            ref = min(ref, 3 if p == 6 else 4)
+        re_ = float(math.exp(rng.uniform(math.log(10), math.log(1600))))
+        ma = float(rng.uniform(0.05, 0.2))
+        steps = int(math.exp(rng.uniform(math.log(100), math.log(5000 if full else 1000))))
+        out.append(dict(problem=prob, stencil=st, refinement=ref, order=p, reynolds=re_, mach=ma, nsteps=steps))
+    return out
+
+
+def run_serving(nat, reqs, precision, seed, load=0.8):
+    """Closed-loop pass (all submitted at t=0) measures the service capacity; the open-loop pass
+    then submits the same requests with Poisson arrivals at `load` x capacity and records each
+    request's latency (submit -> retired)."""
+    import threading
+
+    import numpy as np
+
+    def submit_all(srv, times):
+        ids = [None] * len(reqs)
+        res = [None] * len(reqs)
+
+        def waiter():
+            for k in range(len(reqs)):
+                while ids[k] is None:
+                    time.sleep(1e-4)
+                res[k] = srv.wait(ids[k])
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for k, r in enumerate(reqs):
+            dt = times[k] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids[k] = srv.submit(precision=precision, **r)
+        th.join()
+        return res, time.perf_counter() - t0
+
+    srv = nat.Server(max_active=8, chunk_steps=50, use_graphs=True)
+    try:
+        closed, t_closed = submit_all(srv, [0.0] * len(reqs))
+        st = srv.stats()
+    finally:
+        srv.close()
+    nodes = sum(r["nsteps"] * o["n_dofs"] for r, o in zip(reqs, closed))
+    rate = len(reqs) / t_closed * load
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(reqs)))
+    srv = nat.Server(max_active=8, chunk_steps=50, use_graphs=True)
+    try:
+        opened, t_open = submit_all(srv, list(arrivals))
+    finally:
+        srv.close()
+    lat = np.array([o["latency_ms"] for o in opened])
+    bad = sum(1 for o in closed + opened if o["status"] != 0)
+    return {"requests": len(reqs), "precision": precision, "failed": bad,
+            "closed_loop_s": round(t_closed, 3), "closed_loop_glups": round(nodes / t_closed / 1e9, 4),
+            "arrival_rate_per_s": round(rate, 3), "open_loop_s": round(t_open, 3),
+            "open_loop_glups": round(nodes / t_open / 1e9, 4),
+            "latency_p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "latency_p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "node_updates": int(nodes), "server_busy_ms": round(st["busy_ms"], 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,15 +182,43 @@ def main():
     ap.add_argument("--error-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-full", action="store_true", help="512 requests, full refinement and step ranges")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="profiling run: only the FP32-shifted C2 fused kernel for this many steps")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="profiling run: a plain device copy of 1 GiB, ten times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the hot path runs on a single MI355X; "
               "multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        import torch
+
+        n = 1 << 28  # 2^28 floats = 1 GiB
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": 4 * n, "copy_bytes_written_per_launch": 4 * n,
+                          "launches": 10}))
+        return
+
     import natrium_amd as nat
 
     prob = tgv3d_problem()
+    if args.profile_steps:
+        s32 = make_solver(nat, prob, "fp32")
+        init_tgv(s32, prob)
+        s32.step(args.profile_steps)
+        s32.sync()
+        print(json.dumps({"profile_steps": args.profile_steps, "nodes": s32.n_stored,
+                          "algorithmic_bytes_per_launch": s32.n_stored * 2 * 27 * 4}))
+        return
     Q = 27
     workload = "tgv3d_16c_p4_d3q27"
 
@@ -159,6 +264,12 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(prob, args.cpu_seconds)
 
+    serving = None
+    if not args.no_serve:
+        nreq = 512 if args.serve_full else args.serve_requests
+        reqs = serving_requests(nreq, 2024, args.serve_full)
+        serving = {pr: run_serving(nat, reqs, pr, 2024) for pr in ("fp32", "fp64")}
+
     out = {
         "metric": METRIC,
         "value": round(glups32, 4),
@@ -170,7 +281,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "f64",
+        "storage_dtype": "f32 (populations shifted by w rho0)",
         "data": "synthetic",
         "config": {"workload": workload, "mesh": "[0,2pi]^3 16^3 affine hexes", "order": 4, "stencil": "D3Q27",
                    "Re": 400, "Ma": 0.1, "cfl": 0.4, "nodes_updated_per_step": n_unique,
@@ -185,8 +297,9 @@ def main():
         "ekin_ratio_stokes": math.exp(-6 * prob["viscosity"] * t_end),
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "latency_p50_ms": None,
-        "latency_p99_ms": None,
+        "latency_p50_ms": serving["fp32"]["latency_p50_ms"] if serving else None,
+        "latency_p99_ms": serving["fp32"]["latency_p99_ms"] if serving else None,
+        "serving": serving,
     }
     print(json.dumps(out))
 

@@ -58,39 +58,92 @@ struct StructArgs {
 constexpr int win_n(int c, int t, int p) { return c > 0 ? t + p + 1 : (c < 0 ? t + 1 : t); }
 constexpr int win_lo(int c, int p) { return c > 0 ? -p : 0; }
 
-// Compile-time geometry of pass K of direction AL.
-template <int D, int Q, int P, int TX, int TY, int TZ, int AL, int K>
-struct PassGeo {
-  using L = Lattice<Q>;
-  static constexpr int c0 = L::e[AL][0], c1 = L::e[AL][1], c2 = (D == 3) ? L::e[AL][2] : 0;
-  static constexpr int comp(int a) { return a == 0 ? c0 : (a == 1 ? c1 : c2); }
-  static constexpr int tile(int a) { return a == 0 ? TX : (a == 1 ? TY : TZ); }
-  static constexpr int npass = (c0 != 0) + (c1 != 0) + (c2 != 0);
-  static constexpr int axis_of(int k) {
+// Compile-time geometry of one 1-D pass. A pass reads valid input coordinates
+// (window or tile extents per axis) and writes the valid outputs of each source
+// cell into a compact layout whose extent along the pass axis is the tile's. For a
+// positive component the first and last source cells of a line are partial: the
+// first only feeds the tile's first vertex, the last misses its final output.
+struct PG {
+  int D, P, T[3], c[3], K;
+  int np, A, C;
+  bool first, last;
+  int ext[3];              // valid input extents
+  int pin[3], inoff[3];    // physical input extents and offsets (K > 0)
+  int pout[3], outoff;     // physical output extents, offset of valid outputs along A
+  int NC, O1, O2, E1, E2, items;
+  int sin[3], sout[3];     // physical strides
+  constexpr PG(int D_, int P_, int tx, int ty, int tz, int c0, int c1, int c2, int K_)
+      : D(D_), P(P_), T{tx, ty, tz}, c{c0, c1, c2}, K(K_), np(0), A(-1), C(0), first(K_ == 0), last(false),
+        ext{}, pin{}, inoff{}, pout{}, outoff(0), NC(0), O1(0), O2(0), E1(0), E2(0), items(0), sin{}, sout{} {
+    np = (c[0] != 0) + (c[1] != 0) + (c[2] != 0);
+    A = axis_of(K);
+    C = A >= 0 ? c[A] : 0;
+    last = (K == np - 1);
+    for (int a = 0; a < 3; a++) ext[a] = processed(a, K) ? T[a] : win_n(c[a], T[a], P);
+    for (int a = 0; a < 3; a++) {
+      pin[a] = ext[a];
+      inoff[a] = 0;
+      pout[a] = (a == A) ? T[a] : ext[a];
+    }
+    outoff = 0;
+    if (A >= 0) NC = C > 0 ? T[A] / P + 1 : T[A] / P;
+    O1 = (A == 0) ? 1 : 0;
+    O2 = (A == 2) ? 1 : 2;
+    E1 = ext[O1];
+    E2 = ext[O2];
+    items = E1 * E2 * NC;
+    sin[0] = 1;
+    sin[1] = pin[0];
+    sin[2] = pin[0] * pin[1];
+    sout[0] = 1;
+    sout[1] = pout[0];
+    sout[2] = pout[0] * pout[1];
+  }
+  // pass order z, y, x over the non-zero components
+  constexpr int axis_of(int k) const {
     int n = 0;
     for (int a = 2; a >= 0; a--)
-      if (comp(a) != 0) {
+      if (c[a] != 0) {
         if (n == k) return a;
         n++;
       }
     return -1;
   }
-  static constexpr bool processed(int a) {
-    for (int j = 0; j < K; j++)
+  constexpr bool processed(int a, int k) const {
+    for (int j = 0; j < k; j++)
       if (axis_of(j) == a) return true;
     return false;
   }
-  static constexpr int A = axis_of(K);
-  static constexpr int C = comp(A);
-  static constexpr bool first = (K == 0), last = (K == npass - 1);
-  static constexpr int ext(int a) { return processed(a) ? tile(a) : win_n(comp(a), tile(a), P); }
-  static constexpr int oext(int a) { return a == A ? tile(A) : ext(a); }
-  static constexpr int EX = ext(0), EY = ext(1), EZ = ext(2);
-  static constexpr int OX = oext(0), OY = oext(1), OZ = oext(2);
-  static constexpr int NC = C > 0 ? tile(A) / P + 1 : tile(A) / P;  // source cells covering the tile
-  static constexpr int O1 = (A == 0) ? 1 : 0, O2 = (A == 2) ? 1 : 2;  // other axes, O1 < O2
-  static constexpr int E1 = ext(O1), E2 = ext(O2);
-  static constexpr int items = E1 * E2 * NC;
+  constexpr int out_size() const { return pout[0] * pout[1] * pout[2]; }
+};
+
+template <int D, int Q, int P, int TX, int TY, int TZ, int AL, int K>
+struct PassGeo {
+  static constexpr PG g{D, P, TX, TY, TZ, Lattice<Q>::e[AL][0], Lattice<Q>::e[AL][1],
+                        (D == 3) ? Lattice<Q>::e[AL][2] : 0, K};
+};
+
+// LDS sizes: largest output of pass 0 (b1), pass 1 (b2) and of a last pass (b3).
+template <int D, int Q, int P, int TX, int TY, int TZ>
+struct BufSizes {
+  int b1 = 1, b2 = 1, b3 = 1;
+  constexpr BufSizes() {
+    for (int al = 1; al < Q; al++) {
+      const int c2 = (D == 3) ? Lattice<Q>::e[al][2] : 0;
+      for (int k = 0; k < 3; k++) {
+        const PG g(D, P, TX, TY, TZ, Lattice<Q>::e[al][0], Lattice<Q>::e[al][1], c2, k);
+        if (k >= g.np) break;
+        const int sz = g.out_size();
+        if (g.last) {
+          if (sz > b3) b3 = sz;
+        } else if (k == 0) {
+          if (sz > b1) b1 = sz;
+        } else {
+          if (sz > b2) b2 = sz;
+        }
+      }
+    }
+  }
 };
 
 // Processing order: directions grouped B at a time, groups never mixing
@@ -120,21 +173,29 @@ struct Schedule {
 template <int D, int Q, int B>
 inline constexpr Schedule<D, Q, B> kSchedule{};
 
-// p outputs of one source cell. v: the cell's p+1 values. Output kk is the node at
-// position k = kk + (C > 0) of the source cell; its class is k mod p. Weights are
-// FP64 rows [class][m] (row pitch MAXP+1) read with uniform addresses.
-template <int C, int P>
-__device__ __forceinline__ void cell_outputs(const double* __restrict__ wt, const double (&v)[P + 1],
-                                             double (&out)[P]) {
-#pragma unroll
-  for (int kk = 0; kk < P; kk++) {
-    const int j = (kk + (C > 0 ? 1 : 0)) % P;
-    double acc = 0;
-#pragma unroll
-    for (int m = 0; m <= P; m++) acc += wt[j * (MAXP + 1) + m] * v[m];
-    out[kk] = acc;
+// Exact division by a constant for n < N with one 24-bit multiply and a shift.
+template <unsigned DV, unsigned N>
+struct FastDiv {
+  static constexpr unsigned shift() {
+    for (unsigned s = 8; s < 24; s++) {
+      const unsigned long long m = ((1ull << s) + DV - 1) / DV;
+      if (m >= (1u << 24)) break;
+      bool ok = true;
+      for (unsigned n = 0; n < N && ok; n++) ok = ((unsigned long long)n * m >> s) == n / DV;
+      if (ok) return s;
+    }
+    return 0;
   }
-}
+  static constexpr unsigned S = shift();
+  static constexpr unsigned M = (unsigned)(((1ull << S) + DV - 1) / DV);
+  static_assert((DV & (DV - 1)) == 0 || S > 0, "no 24-bit reciprocal for this divisor");
+  __device__ static __forceinline__ unsigned div(unsigned n) {
+    if constexpr ((DV & (DV - 1)) == 0)
+      return n / DV;
+    else
+      return __umul24(n, M) >> S;
+  }
+};
 
 // Barrier for LDS hand-over only: outstanding global loads stay in flight.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -142,18 +203,18 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 template <int D, int Q, int P, int TX, int TY, int TZ, int B>
 struct StructCfg {
   static constexpr int NT = TX * TY * TZ;
-  static constexpr int WX = TX + P + 1, WY = TY + P + 1, WZ = (D == 3) ? TZ + P + 1 : 1;
-  // first-pass outputs: z done (x, y windows) in 3D, y done in 2D
-  static constexpr int B1 = (D == 3) ? WX * WY * TZ : WX * TY;
-  static constexpr int B2 = (D == 3) ? WX * TY * TZ : 1;
-  static constexpr int PX = TX + 1;  // padded row pitch of the tile output
-  static constexpr int B3 = PX * TY * TZ;
-  // register slots for a first pass: worst case over the pass shapes
-  static constexpr int max3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
-  static constexpr int FIRST_ITEMS =
-      (D == 3) ? max3(WX * WY * (TZ / P + 1), WX * (TY / P + 1) * TZ, (TX / P + 1) * TY * TZ)
-               : max3(WX * (TY / P + 1), (TX / P + 1) * TY, 1);
-  static constexpr int KI = (FIRST_ITEMS + NT - 1) / NT;
+  static constexpr BufSizes<D, Q, P, TX, TY, TZ> BS{};
+  static constexpr int B1 = BS.b1, B2 = BS.b2, B3 = BS.b3;
+  // register slots for a first pass
+  static constexpr int first_items() {
+    int m = 1;
+    for (int al = 1; al < Q; al++) {
+      const PG g(D, P, TX, TY, TZ, Lattice<Q>::e[al][0], Lattice<Q>::e[al][1], (D == 3) ? Lattice<Q>::e[al][2] : 0, 0);
+      if (g.items > m) m = g.items;
+    }
+    return m;
+  }
+  static constexpr int KI = (first_items() + NT - 1) / NT;
 };
 
 template <int D, int Q, int P, typename T, int TX, int TY, int TZ, int B, int WPE>
@@ -168,7 +229,7 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   __shared__ T b3[2][B][Cfg::B3];
   __shared__ int gt[3][TW];  // premultiplied wrapped offsets of window nodes [X0-P, X0+T] per axis
 
-  const int tid = threadIdx.x;
+  const unsigned tid = threadIdx.x;
   // tile index, optionally regrouped so that consecutive tiles (which share halos)
   // run on the same XCD: hardware dispatches workgroup i to XCD i % 8
   int bid = blockIdx.x;
@@ -178,16 +239,14 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
                       (bid / (a.tiles[0] * a.tiles[1])) * TZ};
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
   const int astr[3] = {1, nx, nx * ny};
-  const int alim[3] = {nx, nx * ny, nx * ny * nz};  // n_a * stride_a
   for (int i = tid; i < 3 * TW; i += NT) {
     const int ax = i / TW, k = i % TW;
     const int n = a.n[ax];
     gt[ax][k] = (ax < D) ? (((org[ax] - P + k) % n + n) % n) * astr[ax] : 0;
   }
-  const int ox0 = tid % TX, oy0 = (tid / TX) % TY, oz0 = tid / (TX * TY);
-  const bool active = (org[0] + ox0 < nx) && (org[1] + oy0 < ny) && (org[2] + oz0 < nz);
+  const unsigned ox0 = tid % TX, oy0 = (tid / TX) % TY, oz0 = tid / (TX * TY);
+  const bool active = (org[0] + (int)ox0 < nx) && (org[1] + (int)oy0 < ny) && (org[2] + (int)oz0 < nz);
   const int node = (org[0] + ox0) + nx * ((org[1] + oy0) + ny * (org[2] + oz0));
-  const int myb3 = ox0 + Cfg::PX * (oy0 + TY * oz0);
 
   T fs[Q];
   fs[0] = active ? a.fin[node] : T(0);
@@ -198,27 +257,24 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   auto load_first = [&](auto ALc, auto Bc) {
     constexpr int al = decltype(ALc)::value, b = decltype(Bc)::value;
     if constexpr (al > 0) {
-      using G = PassGeo<D, Q, P, TX, TY, TZ, al, 0>;
-      constexpr int KK = (G::items + NT - 1) / NT;
+      constexpr PG G = PassGeo<D, Q, P, TX, TY, TZ, al, 0>::g;
+      constexpr int KK = (G.items + NT - 1) / NT;
       static_assert(KK <= KI, "first-pass register slots");
       const T* __restrict__ src = a.fin + (int64_t)al * a.stride;
-      int t0 = tid;
+      unsigned t0 = tid;
       asm volatile("" : "+v"(t0));
 #pragma unroll
       for (int k = 0; k < KK; k++) {
-        const int it = t0 + k * NT;
-        if (it < G::items) {
-          const int i1 = it % G::E1, r = it / G::E1;
-          const int i2 = r % G::E2, sc = r / G::E2;
-          const int base = gt[G::O1][i1 + win_lo(G::comp(G::O1), P) + P] +
-                           gt[G::O2][i2 + win_lo(G::comp(G::O2), P) + P];
-          const int g0 = gt[G::A][sc * P + win_lo(G::C, P) + P];
+        const unsigned it = t0 + k * NT;
+        if (it < (unsigned)G.items) {
+          const unsigned r = FastDiv<G.E1, KK * NT>::div(it);
+          const unsigned i1 = it - r * G.E1;
+          const unsigned sc = FastDiv<G.E2, (KK * NT) / G.E1 + 1>::div(r);
+          const unsigned i2 = r - sc * G.E2;
+          const int base = gt[G.O1][i1 + win_lo(G.c[G.O1], P) + P] + gt[G.O2][i2 + win_lo(G.c[G.O2], P) + P];
+          const int* ga = &gt[G.A][sc * P + win_lo(G.C, P) + P];
 #pragma unroll
-          for (int m = 0; m <= P; m++) {
-            int g = g0 + m * astr[G::A];
-            g -= (g >= alim[G::A]) ? alim[G::A] : 0;
-            xin[b][k][m] = src[base + g];
-          }
+          for (int m = 0; m <= P; m++) xin[b][k][m] = src[base + ga[m]];
         }
       }
     }
@@ -228,46 +284,51 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   auto run_pass = [&](auto ALc, auto Kc, auto Bc, auto PARc) {
     constexpr int al = decltype(ALc)::value, kp = decltype(Kc)::value;
     constexpr int b = decltype(Bc)::value, par = decltype(PARc)::value;
-    using G = PassGeo<D, Q, P, TX, TY, TZ, al, kp>;
-    constexpr int KK = (G::items + NT - 1) / NT;
-    const double* wt = wtab + (G::A * MAXSHIFT + (G::C > 0 ? 0 : 1)) * MAXP * (MAXP + 1);
-    constexpr int istr = (G::A == 0) ? 1 : (G::A == 1 ? G::EX : G::EX * G::EY);  // input stride along A
-    int t0 = tid;
+    constexpr PG G = PassGeo<D, Q, P, TX, TY, TZ, al, kp>::g;
+    constexpr int KK = (G.items + NT - 1) / NT;
+    // this pass's weight rows [class][m], uniform addresses: scalar loads
+    const double* __restrict__ wt = wtab + (G.A * MAXSHIFT + (G.C > 0 ? 0 : 1)) * MAXP * (MAXP + 1);
+    unsigned t0 = tid;
     asm volatile("" : "+v"(t0));
 #pragma unroll
     for (int k = 0; k < KK; k++) {
-      const int it = t0 + k * NT;
-      if (it < G::items) {
-        const int i1 = it % G::E1, r = it / G::E1;
-        const int i2 = r % G::E2, sc = r / G::E2;
-        int c[3];
-        c[G::O1] = i1;
-        c[G::O2] = i2;
-        c[G::A] = sc * P;
-        double v[P + 1], o[P];
-        if constexpr (G::first) {
+      const unsigned it = t0 + k * NT;
+      if (it < (unsigned)G.items) {
+        const unsigned r = FastDiv<G.E1, KK * NT>::div(it);
+        const unsigned i1 = it - r * G.E1;
+        const unsigned sc = FastDiv<G.E2, (KK * NT) / G.E1 + 1>::div(r);
+        const unsigned i2 = r - sc * G.E2;
+        double v[P + 1];
+        if constexpr (G.first) {
 #pragma unroll
           for (int m = 0; m <= P; m++) v[m] = (double)xin[b][k][m];
         } else {
           const double* in = (kp == 1) ? b1[b] : b2[b];
-          const int i0 = c[0] + G::EX * (c[1] + G::EY * c[2]);
+          const unsigned i0 = (i1 + G.inoff[G.O1]) * G.sin[G.O1] + (i2 + G.inoff[G.O2]) * G.sin[G.O2] +
+                              sc * (P * G.sin[G.A]);
 #pragma unroll
-          for (int m = 0; m <= P; m++) v[m] = in[i0 + m * istr];
+          for (int m = 0; m <= P; m++) v[m] = in[i0 + m * G.sin[G.A]];
         }
-        cell_outputs<G::C, P>(wt, v, o);
-        constexpr int sh = (G::C > 0) ? 1 - P : 0;
+        double acc[P];
 #pragma unroll
         for (int kk = 0; kk < P; kk++) {
-          const int oa = sc * P + kk + sh;
-          if (oa >= 0 && oa < G::tile(G::A)) {
-            int d[3] = {c[0], c[1], c[2]};
-            d[G::A] = oa;
-            if constexpr (G::last) {
-              b3[par][b][d[0] + Cfg::PX * (d[1] + TY * d[2])] = (T)o[kk];
-            } else {
-              double* out = (kp == 0) ? b1[b] : b2[b];
-              out[d[0] + G::OX * (d[1] + G::OY * d[2])] = o[kk];
-            }
+          const double* __restrict__ w = wt + ((kk + (G.C > 0 ? 1 : 0)) % P) * (MAXP + 1);
+          acc[kk] = w[0] * v[0];
+#pragma unroll
+          for (int m = 1; m <= P; m++) acc[kk] += w[m] * v[m];
+        }
+        // output kk is the node sc*P + kk (+1-P for a positive component) of the tile
+        constexpr int sh = (G.C > 0) ? 1 - P : 0;
+        const int o0 = (int)(i1 * G.sout[G.O1] + i2 * G.sout[G.O2]) + ((int)sc * P + sh) * G.sout[G.A];
+        const bool lo_ok = G.C < 0 || sc != 0;                      // outputs kk < P-1
+        const bool hi_ok = G.C < 0 || sc != (unsigned)(G.NC - 1);   // output kk = P-1
+#pragma unroll
+        for (int kk = 0; kk < P; kk++) {
+          if (kk < P - 1 ? lo_ok : hi_ok) {
+            if constexpr (G.last)
+              b3[par][b][o0 + kk * G.sout[G.A]] = (T)acc[kk];
+            else
+              ((kp == 0) ? b1[b] : b2[b])[o0 + kk * G.sout[G.A]] = acc[kk];
           }
         }
       }
@@ -309,7 +370,10 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
       constexpr int b = decltype(Bc)::value;
       constexpr int al = kSchedule<D, Q, B>.dir[g][b];
       if constexpr (al > 0) {
-        T val = b3[g & 1][b][myb3];
+        constexpr PG G = PassGeo<D, Q, P, TX, TY, TZ, al, kSchedule<D, Q, B>.npass[g] - 1>::g;
+        constexpr unsigned off = (G.A == 0 ? G.outoff : 0) + G.sout[1] * (G.A == 1 ? G.outoff : 0) +
+                                 G.sout[2] * (G.A == 2 ? G.outoff : 0);
+        T val = b3[g & 1][b][off + ox0 + G.sout[1] * oy0 + G.sout[2] * oz0];
         asm volatile("" : "+v"(val));
         fs[al] = val;
       }

