@@ -35,8 +35,24 @@
 // upper periodic faces are copies of the lower ones and are produced at download.
 #include "nat_collide.hpp"
 
-#ifndef NAT_STRUCT_B
-#define NAT_STRUCT_B 3  // directions per group
+// directions per group (FP32 / FP64 storage) and the D3Q27 p=4 tile
+#ifndef NAT_STRUCT_B32
+#define NAT_STRUCT_B32 3
+#endif
+#ifndef NAT_STRUCT_B64
+#define NAT_STRUCT_B64 3
+#endif
+#ifndef NAT_PREFETCH32
+#define NAT_PREFETCH32 1  // groups of first-pass loads in flight (FP32 storage)
+#endif
+#ifndef NAT_PREFETCH64
+#define NAT_PREFETCH64 1
+#endif
+#ifndef NAT_WEIGHTS_LDS
+#define NAT_WEIGHTS_LDS 0
+#endif
+#ifndef NAT_TILE_27_4
+#define NAT_TILE_27_4 16, 8, 8
 #endif
 
 namespace nat {
@@ -50,6 +66,7 @@ struct StructArgs {
   int tiles[3];       // tiles per axis
   int xcd_remap;      // 1: group spatially adjacent tiles on the same XCD
   const double* w;    // [3][MAXSHIFT][MAXP][MAXP+1] FP64 weights
+  int dir_bytes;      // bytes of one direction's array (buffer-load range)
 };
 
 // Source window of a tile along one axis for a velocity component c (pattern
@@ -95,9 +112,11 @@ struct PG {
     sin[0] = 1;
     sin[1] = pin[0];
     sin[2] = pin[0] * pin[1];
+    // the last pass's output rows get one element of padding: its items run along y
+    // with lanes one row apart, and an odd pitch spreads them over the LDS banks
     sout[0] = 1;
-    sout[1] = pout[0];
-    sout[2] = pout[0] * pout[1];
+    sout[1] = pout[0] + (last ? 1 : 0);
+    sout[2] = sout[1] * pout[1];
   }
   // pass order z, y, x over the non-zero components
   constexpr int axis_of(int k) const {
@@ -114,7 +133,7 @@ struct PG {
       if (axis_of(j) == a) return true;
     return false;
   }
-  constexpr int out_size() const { return pout[0] * pout[1] * pout[2]; }
+  constexpr int out_size() const { return sout[2] * pout[2]; }
 };
 
 template <int D, int Q, int P, int TX, int TY, int TZ, int AL, int K>
@@ -197,6 +216,13 @@ struct FastDiv {
   }
 };
 
+// Compile-time tag of a direction group: value = output parity, grp = group index.
+template <int G_>
+struct GroupTag {
+  static constexpr int value = G_ & 1;
+  static constexpr int grp = G_;
+};
+
 // Barrier for LDS hand-over only: outstanding global loads stay in flight.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
@@ -227,7 +253,12 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   __shared__ double b1[B][Cfg::B1];
   __shared__ double b2[B][Cfg::B2];
   __shared__ T b3[2][B][Cfg::B3];
-  __shared__ int gt[3][TW];  // premultiplied wrapped offsets of window nodes [X0-P, X0+T] per axis
+  __shared__ int gt[3][TW];  // byte offsets of the wrapped window nodes [X0-P, X0+T] per axis
+#if NAT_WEIGHTS_LDS
+  // 1-D weights [axis][sign][class][m] in LDS: uniform-address (broadcast) reads that
+  // keep the scalar-load counter out of the LDS waits of the passes
+  __shared__ double wl[3][2][P][P + 1];
+#endif
 
   const unsigned tid = threadIdx.x;
   // tile index, optionally regrouped so that consecutive tiles (which share halos)
@@ -239,10 +270,17 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
                       (bid / (a.tiles[0] * a.tiles[1])) * TZ};
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
   const int astr[3] = {1, nx, nx * ny};
+#if NAT_WEIGHTS_LDS
+  for (int i = tid; i < 3 * 2 * P * (P + 1); i += NT) {
+    const int m = i % (P + 1), r = i / (P + 1);
+    const int j = r % P, sg = (r / P) % 2, ax = r / (2 * P);
+    wl[ax][sg][j][m] = wtab[((ax * MAXSHIFT + sg) * MAXP + j) * (MAXP + 1) + m];
+  }
+#endif
   for (int i = tid; i < 3 * TW; i += NT) {
     const int ax = i / TW, k = i % TW;
     const int n = a.n[ax];
-    gt[ax][k] = (ax < D) ? (((org[ax] - P + k) % n + n) % n) * astr[ax] : 0;
+    gt[ax][k] = (ax < D) ? (((org[ax] - P + k) % n + n) % n) * astr[ax] * (int)sizeof(T) : 0;
   }
   const unsigned ox0 = tid % TX, oy0 = (tid / TX) % TY, oz0 = tid / (TX * TY);
   const bool active = (org[0] + (int)ox0 < nx) && (org[1] + (int)oy0 < ny) && (org[2] + (int)oz0 < nz);
@@ -250,19 +288,46 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
 
   T fs[Q];
   fs[0] = active ? a.fin[node] : T(0);
-  T xin[B][KI][P + 1];
+  // first-pass inputs of the next PF groups (register slot sets, rotating by group)
+  constexpr int PF = (sizeof(T) == 4) ? NAT_PREFETCH32 : NAT_PREFETCH64;
+  T xin[PF][B][KI][P + 1];
+  int xo0[PF][B][KI];         // output base of the first-pass item, computed with its loads
+  unsigned xsc[PF][B][KI];    // its source cell along the pass axis
   __syncthreads();
 
+  // raw buffer loads: 32-bit byte offsets against a per-direction resource (range checked)
+  auto load_t = [](__amdgpu_buffer_rsrc_t r, int off) -> T {
+    if constexpr (sizeof(T) == 4)
+      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    else
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  };
+
+  // The directions of a group start their items at different waves, so that a
+  // pass with fewer items than threads does not leave the same waves idle for all
+  // directions of the group.
+  auto rot = [](unsigned t, int b) -> unsigned {
+    const unsigned off = ((unsigned)(b * NT / B) / 64u) * 64u;
+    const unsigned r = t + off;
+    return r >= (unsigned)NT ? r - NT : r;
+  };
+
   // ---- global loads of a direction's first pass into register slot b
-  auto load_first = [&](auto ALc, auto Bc) {
-    constexpr int al = decltype(ALc)::value, b = decltype(Bc)::value;
+  auto load_first = [&](auto ALc, auto Bc, auto SETc) {
+    constexpr int al = decltype(ALc)::value, b = decltype(Bc)::value, ps = decltype(SETc)::value;
     if constexpr (al > 0) {
       constexpr PG G = PassGeo<D, Q, P, TX, TY, TZ, al, 0>::g;
       constexpr int KK = (G.items + NT - 1) / NT;
       static_assert(KK <= KI, "first-pass register slots");
-      const T* __restrict__ src = a.fin + (int64_t)al * a.stride;
-      unsigned t0 = tid;
-      asm volatile("" : "+v"(t0));
+      // the base is uniform; readfirstlane lets the compiler keep the resource in SGPRs
+      // (readfirstlane returns int: both halves go through uint32_t, no sign extension)
+      const uint64_t pb = (uint64_t)(a.fin + (int64_t)al * a.stride);
+      const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pb >> 32));
+      const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pb);
+      const uint64_t pu = ((uint64_t)phi << 32) | (uint64_t)plo;
+      const __amdgpu_buffer_rsrc_t src = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)pu, (short)0, __builtin_amdgcn_readfirstlane(a.dir_bytes), 0x00020000);
+      const unsigned t0 = rot(tid, b);
 #pragma unroll
       for (int k = 0; k < KK; k++) {
         const unsigned it = t0 + k * NT;
@@ -274,7 +339,10 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
           const int base = gt[G.O1][i1 + win_lo(G.c[G.O1], P) + P] + gt[G.O2][i2 + win_lo(G.c[G.O2], P) + P];
           const int* ga = &gt[G.A][sc * P + win_lo(G.C, P) + P];
 #pragma unroll
-          for (int m = 0; m <= P; m++) xin[b][k][m] = src[base + ga[m]];
+          for (int m = 0; m <= P; m++) xin[ps][b][k][m] = load_t(src, base + ga[m]);
+          constexpr int sh = (G.C > 0) ? 1 - P : 0;
+          xo0[ps][b][k] = (int)(i1 * G.sout[G.O1] + i2 * G.sout[G.O2]) + ((int)sc * P + sh) * G.sout[G.A];
+          xsc[ps][b][k] = sc;
         }
       }
     }
@@ -284,42 +352,50 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   auto run_pass = [&](auto ALc, auto Kc, auto Bc, auto PARc) {
     constexpr int al = decltype(ALc)::value, kp = decltype(Kc)::value;
     constexpr int b = decltype(Bc)::value, par = decltype(PARc)::value;
+    constexpr int ps = decltype(PARc)::grp % PF;  // register slot set of this group
     constexpr PG G = PassGeo<D, Q, P, TX, TY, TZ, al, kp>::g;
     constexpr int KK = (G.items + NT - 1) / NT;
     // this pass's weight rows [class][m], uniform addresses: scalar loads
     const double* __restrict__ wt = wtab + (G.A * MAXSHIFT + (G.C > 0 ? 0 : 1)) * MAXP * (MAXP + 1);
-    unsigned t0 = tid;
-    asm volatile("" : "+v"(t0));
+    const unsigned t0 = rot(tid, b);
 #pragma unroll
     for (int k = 0; k < KK; k++) {
       const unsigned it = t0 + k * NT;
       if (it < (unsigned)G.items) {
-        const unsigned r = FastDiv<G.E1, KK * NT>::div(it);
-        const unsigned i1 = it - r * G.E1;
-        const unsigned sc = FastDiv<G.E2, (KK * NT) / G.E1 + 1>::div(r);
-        const unsigned i2 = r - sc * G.E2;
         double v[P + 1];
+        int o0;
+        unsigned sc;
+        constexpr int sh = (G.C > 0) ? 1 - P : 0;
         if constexpr (G.first) {
 #pragma unroll
-          for (int m = 0; m <= P; m++) v[m] = (double)xin[b][k][m];
+          for (int m = 0; m <= P; m++) v[m] = (double)xin[ps][b][k][m];
+          o0 = xo0[ps][b][k];
+          sc = xsc[ps][b][k];
         } else {
+          const unsigned r = FastDiv<G.E1, KK * NT>::div(it);
+          const unsigned i1 = it - r * G.E1;
+          sc = FastDiv<G.E2, (KK * NT) / G.E1 + 1>::div(r);
+          const unsigned i2 = r - sc * G.E2;
           const double* in = (kp == 1) ? b1[b] : b2[b];
           const unsigned i0 = (i1 + G.inoff[G.O1]) * G.sin[G.O1] + (i2 + G.inoff[G.O2]) * G.sin[G.O2] +
                               sc * (P * G.sin[G.A]);
 #pragma unroll
           for (int m = 0; m <= P; m++) v[m] = in[i0 + m * G.sin[G.A]];
+          o0 = (int)(i1 * G.sout[G.O1] + i2 * G.sout[G.O2]) + ((int)sc * P + sh) * G.sout[G.A];
         }
         double acc[P];
 #pragma unroll
         for (int kk = 0; kk < P; kk++) {
+#if NAT_WEIGHTS_LDS
+          const double* w = &wl[G.A][G.C > 0 ? 0 : 1][(kk + (G.C > 0 ? 1 : 0)) % P][0];
+#else
           const double* __restrict__ w = wt + ((kk + (G.C > 0 ? 1 : 0)) % P) * (MAXP + 1);
+#endif
           acc[kk] = w[0] * v[0];
 #pragma unroll
           for (int m = 1; m <= P; m++) acc[kk] += w[m] * v[m];
         }
         // output kk is the node sc*P + kk (+1-P for a positive component) of the tile
-        constexpr int sh = (G.C > 0) ? 1 - P : 0;
-        const int o0 = (int)(i1 * G.sout[G.O1] + i2 * G.sout[G.O2]) + ((int)sc * P + sh) * G.sout[G.A];
         const bool lo_ok = G.C < 0 || sc != 0;                      // outputs kk < P-1
         const bool hi_ok = G.C < 0 || sc != (unsigned)(G.NC - 1);   // output kk = P-1
 #pragma unroll
@@ -336,25 +412,30 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   };
 
   constexpr int NG = kSchedule<D, Q, B>.ngroups;
-  static_for<0, B>([&](auto Bc) {
-    constexpr int b = decltype(Bc)::value;
-    load_first(std::integral_constant<int, kSchedule<D, Q, B>.dir[0][b]>{}, Bc);
+  static_for<0, (PF < NG ? PF : NG)>([&](auto Gc) {
+    constexpr int g = decltype(Gc)::value;
+    static_for<0, B>([&](auto Bc) {
+      constexpr int b = decltype(Bc)::value;
+      load_first(std::integral_constant<int, kSchedule<D, Q, B>.dir[g][b]>{}, Bc, std::integral_constant<int, g % PF>{});
+    });
   });
 
   static_for<0, NG>([&](auto Gc) {
     constexpr int g = decltype(Gc)::value;
     constexpr int np = kSchedule<D, Q, B>.npass[g];
-    using PAR = std::integral_constant<int, g & 1>;
+    using PAR = GroupTag<g>;
     // first passes (register inputs)
     static_for<0, B>([&](auto Bc) {
       constexpr int al = kSchedule<D, Q, B>.dir[g][decltype(Bc)::value];
       if constexpr (al > 0) run_pass(std::integral_constant<int, al>{}, std::integral_constant<int, 0>{}, Bc, PAR{});
     });
-    // next group's inputs are in flight during the remaining passes
-    if constexpr (g + 1 < NG) {
+    // the inputs of group g + PF go into the slot set just consumed; they are in flight
+    // during the remaining passes of this group and all of the next PF - 1 groups
+    if constexpr (g + PF < NG) {
       static_for<0, B>([&](auto Bc) {
         constexpr int b = decltype(Bc)::value;
-        load_first(std::integral_constant<int, kSchedule<D, Q, B>.dir[g + 1][b]>{}, Bc);
+        load_first(std::integral_constant<int, kSchedule<D, Q, B>.dir[g + PF][b]>{}, Bc,
+                   std::integral_constant<int, g % PF>{});
       });
     }
     lds_barrier();
@@ -381,7 +462,7 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   });
 
   if (!active) return;
-  bgk_collide_store<Q, T>(fs, ca, node, write_macro != 0, a.fout, a.stride);
+  bgk_collide_store_pairs<Q, T>(fs, ca, node, write_macro != 0, a.fout, a.stride);
 }
 
 template <int D, int Q, int P, typename T, int TX, int TY, int TZ>
@@ -391,7 +472,7 @@ static int launch_struct(nat_ctx* ctx, StructArgs<T> a, const CollideArgs<T>& ca
   a.tiles[2] = D == 3 ? (a.n[2] + TZ - 1) / TZ : 1;
   const int nb = a.tiles[0] * a.tiles[1] * a.tiles[2];
   a.xcd_remap = (nb % 8 == 0) ? 1 : 0;
-  constexpr int B = NAT_STRUCT_B;
+  constexpr int B = sizeof(T) == 4 ? NAT_STRUCT_B32 : NAT_STRUCT_B64;
   constexpr int WPE = 4;  // 16 waves per CU: one 1024-thread tile, up to 128 VGPRs
   hipLaunchKernelGGL((k_sl_struct<D, Q, P, T, TX, TY, TZ, B, WPE>), dim3(nb), dim3(TX * TY * TZ), 0, ctx->stream,
                      a, a.w, ca, write_macro);
@@ -404,13 +485,14 @@ static int dispatch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideAr
   const int Q = ctx->Q, P = ctx->p;
 #define NAT_S3(QQ, PP, TX, TY, TZ) \
   if (Q == QQ && P == PP) return launch_struct<3, QQ, PP, T, TX, TY, TZ>(ctx, a, ca, wm);
+#define NAT_S3X(...) NAT_S3(__VA_ARGS__)
 #define NAT_S2(QQ, PP, TX, TY) \
   if (Q == QQ && P == PP) return launch_struct<2, QQ, PP, T, TX, TY, 1>(ctx, a, ca, wm);
   if (ctx->D == 3) {
     NAT_S3(27, 1, 16, 8, 8)
     NAT_S3(27, 2, 16, 8, 8)
     NAT_S3(27, 3, 12, 12, 6)
-    NAT_S3(27, 4, 16, 8, 8)
+    NAT_S3X(27, 4, NAT_TILE_27_4)
     NAT_S3(27, 6, 12, 12, 6)
     NAT_S3(19, 1, 16, 8, 8)
     NAT_S3(19, 2, 16, 8, 8)
@@ -425,6 +507,7 @@ static int dispatch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideAr
     NAT_S2(9, 6, 24, 24)
   }
 #undef NAT_S3
+#undef NAT_S3X
 #undef NAT_S2
   ctx->err = "structured SL kernel not instantiated for this (stencil, order)";
   return NAT_ERR_UNSUPPORTED;
@@ -439,6 +522,11 @@ static int launch_typed(nat_ctx* ctx, int write_macro) {
   a.stride = ctx->stride;
   for (int i = 0; i < 3; i++) a.n[i] = ctx->nu[i];
   a.w = (const double*)ctx->ax_w;
+  if ((int64_t)ctx->stride * (int64_t)sizeof(T) > INT32_MAX) {
+    ctx->err = "structured backend: one direction's array exceeds 2 GiB";
+    return NAT_ERR_UNSUPPORTED;
+  }
+  a.dir_bytes = (int)(ctx->stride * (int64_t)sizeof(T));
   CollideArgs<T> ca;
   ca.omega = (T)(1.0 / ctx->tau);
   ca.omega64 = 1.0 / ctx->tau;

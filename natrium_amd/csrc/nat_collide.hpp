@@ -169,4 +169,83 @@ __device__ __forceinline__ void bgk_collide_store(const T (&f)[Q], const Collide
   }
 }
 
+// Opposite direction of i in the integer velocity table.
+template <int Q>
+constexpr int opposite(int i) {
+  for (int j = 0; j < Q; j++)
+    if (Lattice<Q>::e[j][0] == -Lattice<Q>::e[i][0] && Lattice<Q>::e[j][1] == -Lattice<Q>::e[i][1] &&
+        Lattice<Q>::e[j][2] == -Lattice<Q>::e[i][2])
+      return j;
+  return -1;
+}
+
+// bgk_collide_store with the equilibrium of each opposite pair (i, i') formed from
+// shared even and odd parts: f_eq(i/i') = w (E +- O), E = drho + rho (ue^2/2 - 3u.u/2),
+// O = rho ue (same values as the direction-by-direction form up to FP64 rounding).
+// Stores go through a uniform direction base and a 32-bit byte offset.
+template <int Q, typename T>
+__device__ __forceinline__ void bgk_collide_store_pairs(const T (&f)[Q], const CollideArgs<T>& ca, int64_t node,
+                                                        bool write_macro, T* __restrict__ fout, int64_t stride) {
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  double fd[Q];
+#pragma unroll
+  for (int i = 0; i < Q; i++) fd[i] = (double)f[i];
+  double drho = 0;
+#pragma unroll
+  for (int i = 0; i < Q; i++) drho += fd[i];
+  double j[3] = {0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (L::e[i][d] > 0) j[d] += fd[i];
+      if (L::e[i][d] < 0) j[d] -= fd[i];
+    }
+  }
+  const double rho = (double)ca.rho0 + drho;
+  if (!(rho >= 1e-10)) atomicOr(ca.status, 1);
+  const double inv = 1.0 / rho;
+  double u[3], u3[3];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    u[d] = j[d] * inv;
+    u3[d] = 3.0 * u[d];
+  }
+  double uu = 0;
+#pragma unroll
+  for (int d = 0; d < D; d++) uu += u[d] * u[d];
+  const double base = drho - rho * (1.5 * uu);  // even part without the ue^2 term
+  const double hr = 0.5 * rho;
+  const double om = ca.omega64;
+  const uint32_t nb = (uint32_t)node * (uint32_t)sizeof(T);
+  auto store = [&](int i, double v) {
+    char* p = (char*)(fout + (int64_t)i * stride);
+    *(T*)(p + nb) = (T)v;
+  };
+  store(0, fd[0] - om * (fd[0] - L::w[0] * base));
+  static_for<1, Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr int o = opposite<Q>(i);
+    if constexpr (o > i) {
+      double ue = 0;
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        if (L::e[i][d] > 0) ue += u3[d];
+        if (L::e[i][d] < 0) ue -= u3[d];
+      }
+      const double E = base + hr * (ue * ue);
+      const double O = rho * ue;
+      const double wE = L::w[i] * E, wO = L::w[i] * O;
+      store(i, fd[i] - om * (fd[i] - (wE + wO)));
+      store(o, fd[o] - om * (fd[o] - (wE - wO)));
+    }
+  });
+  if (write_macro && ca.rho) {
+    ca.rho[node] = (T)rho;
+#pragma unroll
+    for (int d = 0; d < D; d++) ca.u[d * ca.mstride + node] = (T)(u[d] * ca.scaling);
+  }
+}
+
 }  // namespace nat
