@@ -97,6 +97,78 @@ def cpu_baseline(prob, seconds):
                       f"{o.n} FE_Q DoFs, {el:.1f} s wall, OpenMP {threads} threads; assembly untimed"}
 
 
+def extra_configs(nat, steps_sl=100, steps_err=500):
+    """Beside the headline: the other hot-path configurations, FP32-shifted and FP64 GLUPS on
+    the device clock (HIP events), and FP32-shifted vs FP64 velocity errors.
+
+    C3: [0,2pi]^3, 24^3 hexes with interior vertices moved by U(-0.1h, 0.1h) (seed 1234), p=3,
+        D3Q19, Ma=0.05, nu=0.01 (Re=100 with unit length), seeded divergence-free Fourier
+        velocity (8 modes, amplitude 0.05 c_s): general SL backend (389,017 DoFs) and SEDG with
+        RK5 (884,736 DG nodes).
+    C4: unit square, 128^2 quads, p=2, D2Q25H compressible f/g, Riemann quadrant states
+        (seed 7); periodic box (walls are not part of this build).
+    """
+    import numpy as np
+
+    from natrium_amd.problems import (box_vertices, divergence_free_field, lbm_parameters, perturbed_vertices,
+                                      riemann_quadrant_fields, riemann_states)
+
+    out = {}
+    cs, s = lbm_parameters(1.0, 0.05)
+    L = 2 * math.pi
+    v3 = perturbed_vertices(3, [24] * 3, [0] * 3, [L] * 3, amplitude=0.1, seed=1234)
+
+    def run(make, init, steps, errsteps):
+        res = {}
+        sols = {}
+        for prec in ("fp32", "fp64"):
+            sol = make(prec)
+            init(sol)
+            sol.step(2)
+            ms = sol.step_timed(steps)
+            res[prec + "_glups"] = round(sol.n * steps / (ms / 1e3) / 1e9, 4)
+            res[prec + "_ms_per_step"] = round(ms / steps, 4)
+            sols[prec] = sol
+        res["n_dofs"] = sols["fp64"].n
+        res["fp32_over_fp64"] = round(res["fp32_glups"] / res["fp64_glups"], 3)
+        if errsteps:
+            a32, a64 = make("fp32"), make("fp64")
+            init(a32)
+            init(a64)
+            a32.step(errsteps)
+            a64.step(errsteps)
+            _, u32, _ = a32.macro()
+            _, u64, _ = a64.macro()
+            res["l2_err_u_fp32_vs_fp64"] = float(np.sqrt(((u32 - u64) ** 2).sum() / (u64 ** 2).sum()))
+            res["l2_err_steps"] = errsteps
+        return res
+
+    def init3(sol):
+        x = sol.coords()
+        u = divergence_free_field(x, modes=8, amplitude=0.05 * cs, seed=1234)
+        sol.init_equilibrium(np.ones(sol.n), u)
+        sol.collide()
+
+    mk_sl = lambda prec: nat.Solver("D3Q19", 3, [24] * 3, v3, cartesian=False, scaling=s, viscosity=0.01, cfl=0.4,
+                                    precision=prec)
+    out["c3_sl_general"] = run(mk_sl, init3, steps_sl, steps_err)
+    mk_dg = lambda prec: nat.Solver("D3Q19", 3, [24] * 3, v3, cartesian=False, scaling=s, viscosity=0.01, cfl=0.4,
+                                    precision=prec, scheme="SEDG")
+    out["c3_sedg_rk5"] = run(mk_dg, init3, max(steps_sl // 5, 10), steps_err // 5)
+
+    v4 = box_vertices(2, [128, 128], [0, 0], [1, 1])
+
+    def init4(sol):
+        rho, u, T = riemann_quadrant_fields(sol.coords(), riemann_states(7))
+        sol.init_equilibrium(rho, u, T)
+        sol.collide()
+
+    mk_c4 = lambda prec: nat.Solver("D2Q25H", 2, [128, 128], v4, scaling=2.0, viscosity=1e-3, cfl=0.4,
+                                    compressible=True, gamma=1.4, precision=prec)
+    out["c4_compressible"] = run(mk_c4, init4, steps_sl, 0)
+    return out
+
+
 def serving_requests(n, seed, full):
     """Seeded serving mix: 2D/3D Taylor-Green or a shock tube on a 2D strip, refinement 3-5,
     p in {2,3,4,6}, Re log-uniform [10,1600], Ma U[0.05,0.2], steps log-uniform [100,5000].
@@ -185,6 +257,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-full", action="store_true", help="512 requests, full refinement and step ranges")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the C3/C4 side measurements")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="profiling run: only the FP32-shifted C2 fused kernel for this many steps")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -264,6 +337,8 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(prob, args.cpu_seconds)
 
+    extra = None if args.no_extra else extra_configs(nat)
+
     serving = None
     if not args.no_serve:
         nreq = 512 if args.serve_full else args.serve_requests
@@ -300,6 +375,7 @@ def main():
         "latency_p50_ms": serving["fp32"]["latency_p50_ms"] if serving else None,
         "latency_p99_ms": serving["fp32"]["latency_p99_ms"] if serving else None,
         "serving": serving,
+        "other_configs": extra,
     }
     print(json.dumps(out))
 

@@ -48,9 +48,6 @@
 #ifndef NAT_PREFETCH64
 #define NAT_PREFETCH64 1
 #endif
-#ifndef NAT_WEIGHTS_LDS
-#define NAT_WEIGHTS_LDS 0
-#endif
 #ifndef NAT_TILE_27_4
 #define NAT_TILE_27_4 16, 8, 8
 #endif
@@ -254,11 +251,6 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   __shared__ double b2[B][Cfg::B2];
   __shared__ T b3[2][B][Cfg::B3];
   __shared__ int gt[3][TW];  // byte offsets of the wrapped window nodes [X0-P, X0+T] per axis
-#if NAT_WEIGHTS_LDS
-  // 1-D weights [axis][sign][class][m] in LDS: uniform-address (broadcast) reads that
-  // keep the scalar-load counter out of the LDS waits of the passes
-  __shared__ double wl[3][2][P][P + 1];
-#endif
 
   const unsigned tid = threadIdx.x;
   // tile index, optionally regrouped so that consecutive tiles (which share halos)
@@ -270,13 +262,6 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
                       (bid / (a.tiles[0] * a.tiles[1])) * TZ};
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
   const int astr[3] = {1, nx, nx * ny};
-#if NAT_WEIGHTS_LDS
-  for (int i = tid; i < 3 * 2 * P * (P + 1); i += NT) {
-    const int m = i % (P + 1), r = i / (P + 1);
-    const int j = r % P, sg = (r / P) % 2, ax = r / (2 * P);
-    wl[ax][sg][j][m] = wtab[((ax * MAXSHIFT + sg) * MAXP + j) * (MAXP + 1) + m];
-  }
-#endif
   for (int i = tid; i < 3 * TW; i += NT) {
     const int ax = i / TW, k = i % TW;
     const int n = a.n[ax];
@@ -383,17 +368,15 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
           for (int m = 0; m <= P; m++) v[m] = in[i0 + m * G.sin[G.A]];
           o0 = (int)(i1 * G.sout[G.O1] + i2 * G.sout[G.O2]) + ((int)sc * P + sh) * G.sout[G.A];
         }
+        // the P accumulations are independent chains: interleave them (m outer)
         double acc[P];
 #pragma unroll
-        for (int kk = 0; kk < P; kk++) {
-#if NAT_WEIGHTS_LDS
-          const double* w = &wl[G.A][G.C > 0 ? 0 : 1][(kk + (G.C > 0 ? 1 : 0)) % P][0];
-#else
-          const double* __restrict__ w = wt + ((kk + (G.C > 0 ? 1 : 0)) % P) * (MAXP + 1);
-#endif
-          acc[kk] = w[0] * v[0];
+        for (int m = 0; m <= P; m++) {
 #pragma unroll
-          for (int m = 1; m <= P; m++) acc[kk] += w[m] * v[m];
+          for (int kk = 0; kk < P; kk++) {
+            const double w = wt[((kk + (G.C > 0 ? 1 : 0)) % P) * (MAXP + 1) + m];
+            acc[kk] = (m == 0) ? w * v[0] : acc[kk] + w * v[m];
+          }
         }
         // output kk is the node sc*P + kk (+1-P for a positive component) of the tile
         const bool lo_ok = G.C < 0 || sc != 0;                      // outputs kk < P-1
