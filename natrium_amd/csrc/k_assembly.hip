@@ -353,8 +353,12 @@ namespace nat {
 // FP32 departure records: code = cell * (p+1)^D + anchor (the GLL node nearest
 // to the departure point, per axis, x fastest) and dxi = xi - x_anchor per axis.
 // The small offset keeps FP32 relative precision where the weights need it.
+// Compact records for FP32 storage: the departure cell and nearest GLL node packed in
+// one int32, and the unit-coordinate offset from that node in fixed point (2^-30
+// units: |offset| <= 1/2, resolution 9.3e-10), from which the kernel rebuilds FP64
+// weights.
 __global__ void k_records_f32(const int* __restrict__ cell, const double* __restrict__ xi, int64_t nrec, int D,
-                              int p, GllTab g, int* __restrict__ code, float* __restrict__ dxi) {
+                              int p, GllTab g, int* __restrict__ code, int* __restrict__ dxi) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrec) return;
   const int c = cell[r];
@@ -364,7 +368,7 @@ __global__ void k_records_f32(const int* __restrict__ cell, const double* __rest
     int best = 0;
     for (int m = 1; m <= p; m++)
       if (fabs(x - g.x[m]) < fabs(x - g.x[best])) best = m;
-    dxi[r * D + d] = (float)(x - g.x[best]);
+    dxi[r * D + d] = (int)llrint((x - g.x[best]) * 1073741824.0);
     lin += best * mult;
     mult *= p + 1;
   }
@@ -378,7 +382,7 @@ int nat_launch_records_f32(nat_ctx* ctx, int64_t nrec) {
   nat::gll_points(ctx->p, g.x, w);
   const int bs = 256;
   hipLaunchKernelGGL(nat::k_records_f32, dim3((unsigned)((nrec + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
-                     ctx->dep_cell, ctx->dep_xi64, nrec, ctx->D, ctx->p, g, ctx->dep_code32, ctx->dep_xi32);
+                     ctx->dep_cell, ctx->dep_xi64, nrec, ctx->D, ctx->p, g, ctx->dep_code32, ctx->dep_off32);
   NAT_HIP(hipGetLastError());
   return 0;
 }

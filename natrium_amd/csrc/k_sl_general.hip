@@ -4,15 +4,17 @@
 // The assembled reference operator stores, per (DoF, direction), the shape values
 // of the departure cell at the departure point, dropping |w| < 1e-10
 // (advection/SemiLagrangian.cpp:457-503, SemiLagrangianTools.cpp:15-64). Here each
-// (direction, DoF) keeps only its departure cell and unit coordinates (assembled in
-// FP64 on the GPU); the (p+1)^D tensor-product Lagrange weights are re-evaluated in
-// registers every step with the same cut-off. That is 4 + D*sizeof(T) bytes per row
-// instead of (p+1)^D stored weights plus column indices.
+// (direction, DoF) keeps only its departure point (assembled in FP64 on the GPU);
+// the (p+1)^D tensor-product Lagrange weights are re-evaluated in FP64 registers
+// every step with the same cut-off. Records: FP64 storage keeps the cell and FP64
+// unit coordinates (4 + 8D bytes); FP32 storage keeps the cell and nearest GLL node
+// packed in one int32 and the offsets from that node in 2^-30 fixed point (4 + 4D
+// bytes), instead of (p+1)^D stored weights plus column indices per row.
 #include "nat_collide.hpp"
 
 namespace nat {
 
-template <typename T, typename XT>
+template <typename T>
 struct GeneralArgs {
   const T* fin;
   T* fout;
@@ -20,30 +22,50 @@ struct GeneralArgs {
   int64_t n;          // FE_Q DoFs
   int nfe[3];         // FE_Q lattice nodes per axis
   int N[3];           // cells per axis
-  const int* cell;    // FP64: [(alpha-1) * n + i] departure cell (-1: identity row)
-  const XT* xi;       // FP64: unit coordinates; FP32: offsets from the anchor node, D per record
-  const int* code;    // FP32: cell * (p+1)^D + anchor (-1: identity row)
-  T gll[MAXP + 1];
-  T inv_den[MAXP + 1];
-  T dgl[MAXP + 1][MAXP + 1];  // x_a - x_k
+  const int* cell;    // FP64 storage: [(alpha-1) * n + i] departure cell (-1: identity row)
+  const double* xi;   //   unit coordinates, D per record
+  const int* code;    // FP32 storage: cell * (p+1)^D + anchor node (-1: identity row)
+  const int* off;     //   fixed-point offsets from the anchor node (2^-30 units), D per record
+  double gll[MAXP + 1];
+  double inv_den[MAXP + 1];
+  double dgl[MAXP + 1][MAXP + 1];  // x_a - x_k
 };
 
-template <int P, typename T>
-__device__ __forceinline__ void lagrange_all(const T (&x)[MAXP + 1], const T (&inv)[MAXP + 1], T xi, T (&v)[P + 1]) {
+// Lagrange values at x_a + d (anchor a), in FP64: prod_{k != j} (x_a - x_k + d) / den_j
+template <int P>
+__device__ __forceinline__ void basis_at(const double (&dgl)[MAXP + 1][MAXP + 1], const double (&inv)[MAXP + 1],
+                                         int a, double d, double (&v)[P + 1]) {
 #pragma unroll
   for (int j = 0; j <= P; j++) {
-    T s = inv[j];
+    double t = inv[j];
 #pragma unroll
-    for (int m = 0; m <= P; m++)
-      if (m != j) s *= (xi - x[m]);
-    v[j] = s;
+    for (int k = 0; k <= P; k++)
+      if (k != j) t *= (dgl[a][k] + d);
+    v[j] = t;
   }
 }
 
-template <int D, int Q, int P, typename T, typename XT>
-__global__ __launch_bounds__(256) void k_sl_general(GeneralArgs<T, XT> a, CollideArgs<T> ca, int write_macro) {
+template <int P>
+__device__ __forceinline__ void basis_xi(const double (&x)[MAXP + 1], const double (&inv)[MAXP + 1], double xi,
+                                         double (&v)[P + 1]) {
+#pragma unroll
+  for (int j = 0; j <= P; j++) {
+    double t = inv[j];
+#pragma unroll
+    for (int m = 0; m <= P; m++)
+      if (m != j) t *= (xi - x[m]);
+    v[j] = t;
+  }
+}
+
+// Weights, cut-off and arithmetic in FP64 for both storage precisions (FP32 weights
+// rounded the same way at every step bias the FP32-shifted result).
+template <int D, int Q, int P, typename T>
+__global__ __launch_bounds__(256) void k_sl_general(GeneralArgs<T> a, CollideArgs<T> ca, int write_macro) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
+  constexpr int NP = P + 1;
+  constexpr int NL = (D == 3) ? NP * NP * NP : NP * NP;
   T fs[Q];
   fs[0] = a.fin[i];
   const int n0 = a.nfe[0], n1 = a.nfe[1];
@@ -51,115 +73,74 @@ __global__ __launch_bounds__(256) void k_sl_general(GeneralArgs<T, XT> a, Collid
   for (int al = 1; al < Q; al++) {
     const int64_t r = (int64_t)(al - 1) * a.n + i;
     const T* src = a.fin + (int64_t)al * a.stride;
+    int c;
+    double lx[NP], ly[NP], lz[NP];
     if constexpr (sizeof(T) == 8) {
-      const int c = a.cell[r];
+      c = a.cell[r];
       if (c < 0) {  // tracker life-time limit: identity row
         fs[al] = src[i];
         continue;
       }
-      T lx[P + 1], ly[P + 1], lz[P + 1];
-      lagrange_all<P, T>(a.gll, a.inv_den, (T)a.xi[r * D + 0], lx);
-      lagrange_all<P, T>(a.gll, a.inv_den, (T)a.xi[r * D + 1], ly);
-      if constexpr (D == 3) lagrange_all<P, T>(a.gll, a.inv_den, (T)a.xi[r * D + 2], lz);
-      const int cx = c % a.N[0];
-      const int cy = (c / a.N[0]) % a.N[1];
-      const int cz = (D == 3) ? c / (a.N[0] * a.N[1]) : 0;
-      const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P + (int64_t)n1 * ((int64_t)cz * P));
-      T acc = 0;
-      if constexpr (D == 2) {
-#pragma unroll
-        for (int my = 0; my <= P; my++)
-#pragma unroll
-          for (int mx = 0; mx <= P; mx++) {
-            const T w = lx[mx] * ly[my];
-            if (!(fabs(w) < T(1e-10))) acc += w * src[base + mx + (int64_t)n0 * my];
-          }
-      } else {
-#pragma unroll
-        for (int mz = 0; mz <= P; mz++)
-#pragma unroll
-          for (int my = 0; my <= P; my++)
-#pragma unroll
-            for (int mx = 0; mx <= P; mx++) {
-              const T w = lx[mx] * ly[my] * lz[mz];
-              if (!(fabs(w) < T(1e-10))) acc += w * src[base + mx + (int64_t)n0 * (my + (int64_t)n1 * mz)];
-            }
-      }
-      fs[al] = acc;
+      basis_xi<P>(a.gll, a.inv_den, a.xi[r * D + 0], lx);
+      basis_xi<P>(a.gll, a.inv_den, a.xi[r * D + 1], ly);
+      if constexpr (D == 3) basis_xi<P>(a.gll, a.inv_den, a.xi[r * D + 2], lz);
     } else {
-      // FP32: weights from the anchor offsets, delta form around the anchor node
-      // (its own term is w_a (f_a - f_a) = 0, so it needs no special case)
       const int code = a.code[r];
       if (code < 0) {
         fs[al] = src[i];
         continue;
       }
-      constexpr int NP = P + 1;
-      constexpr int NL = (D == 3) ? NP * NP * NP : NP * NP;
-      const int c = code / NL;
+      c = code / NL;
       const int lin = code - c * NL;
       const int ax = lin % NP, ay = (lin / NP) % NP, az = (D == 3) ? lin / (NP * NP) : 0;
-      T lx[NP], ly[NP], lz[NP];
-      auto basis = [&](int anc, T d, T(&v)[NP]) {
+      constexpr double q = 1.0 / 1073741824.0;
+      basis_at<P>(a.dgl, a.inv_den, ax, (double)a.off[r * D + 0] * q, lx);
+      basis_at<P>(a.dgl, a.inv_den, ay, (double)a.off[r * D + 1] * q, ly);
+      if constexpr (D == 3) basis_at<P>(a.dgl, a.inv_den, az, (double)a.off[r * D + 2] * q, lz);
+    }
+    const int cx = c % a.N[0];
+    const int cy = (c / a.N[0]) % a.N[1];
+    const int cz = (D == 3) ? c / (a.N[0] * a.N[1]) : 0;
+    const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P + (int64_t)n1 * ((int64_t)cz * P));
+    double acc = 0;
+    if constexpr (D == 2) {
 #pragma unroll
-        for (int j = 0; j <= P; j++) {
-          T t = a.inv_den[j];
+      for (int my = 0; my <= P; my++)
 #pragma unroll
-          for (int k = 0; k <= P; k++)
-            if (k != j) t *= (a.dgl[anc][k] + d);
-          v[j] = t;
+        for (int mx = 0; mx <= P; mx++) {
+          const double w = lx[mx] * ly[my];
+          if (!(fabs(w) < 1e-10)) acc += w * (double)src[base + mx + (int64_t)n0 * my];
         }
-      };
-      basis(ax, (T)a.xi[r * D + 0], lx);
-      basis(ay, (T)a.xi[r * D + 1], ly);
-      if constexpr (D == 3) basis(az, (T)a.xi[r * D + 2], lz);
-      const int cx = c % a.N[0];
-      const int cy = (c / a.N[0]) % a.N[1];
-      const int cz = (D == 3) ? c / (a.N[0] * a.N[1]) : 0;
-      const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P + (int64_t)n1 * ((int64_t)cz * P));
-      const T fa = src[base + ax + (int64_t)n0 * (ay + (int64_t)n1 * az)];
-      T acc = 0;
-      if constexpr (D == 2) {
+    } else {
+#pragma unroll
+      for (int mz = 0; mz <= P; mz++)
 #pragma unroll
         for (int my = 0; my <= P; my++)
 #pragma unroll
           for (int mx = 0; mx <= P; mx++) {
-            const T w = lx[mx] * ly[my];
-            if (!(fabs(w) < T(1e-10))) acc += w * (src[base + mx + (int64_t)n0 * my] - fa);
+            const double w = lx[mx] * ly[my] * lz[mz];
+            if (!(fabs(w) < 1e-10)) acc += w * (double)src[base + mx + (int64_t)n0 * (my + (int64_t)n1 * mz)];
           }
-      } else {
-#pragma unroll
-        for (int mz = 0; mz <= P; mz++)
-#pragma unroll
-          for (int my = 0; my <= P; my++)
-#pragma unroll
-            for (int mx = 0; mx <= P; mx++) {
-              const T w = lx[mx] * ly[my] * lz[mz];
-              if (!(fabs(w) < T(1e-10))) acc += w * (src[base + mx + (int64_t)n0 * (my + (int64_t)n1 * mz)] - fa);
-            }
-      }
-      fs[al] = fa + acc;
     }
+    fs[al] = (T)acc;
   }
-  bgk_collide<Q, T>(fs, ca, i, write_macro != 0);
-#pragma unroll
-  for (int q = 0; q < Q; q++) a.fout[(int64_t)q * a.stride + i] = fs[q];
+  bgk_collide_store<Q, T>(fs, ca, i, write_macro != 0, a.fout, a.stride);
 }
 
-template <int D, int Q, int P, typename T, typename XT>
-static int launch_general(nat_ctx* ctx, const GeneralArgs<T, XT>& a, const CollideArgs<T>& ca, int wm) {
+template <int D, int Q, int P, typename T>
+static int launch_general(nat_ctx* ctx, const GeneralArgs<T>& a, const CollideArgs<T>& ca, int wm) {
   const int bs = 256;
-  hipLaunchKernelGGL((k_sl_general<D, Q, P, T, XT>), dim3((unsigned)((a.n + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
+  hipLaunchKernelGGL((k_sl_general<D, Q, P, T>), dim3((unsigned)((a.n + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
                      a, ca, wm);
   NAT_HIP(hipGetLastError());
   return 0;
 }
 
-template <typename T, typename XT>
-static int dispatch_general(nat_ctx* ctx, const GeneralArgs<T, XT>& a, const CollideArgs<T>& ca, int wm) {
+template <typename T>
+static int dispatch_general(nat_ctx* ctx, const GeneralArgs<T>& a, const CollideArgs<T>& ca, int wm) {
   const int Q = ctx->Q, P = ctx->p;
 #define NAT_G(DD, QQ, PP) \
-  if (Q == QQ && P == PP) return launch_general<DD, QQ, PP, T, XT>(ctx, a, ca, wm);
+  if (Q == QQ && P == PP) return launch_general<DD, QQ, PP, T>(ctx, a, ca, wm);
   NAT_G(2, 9, 1)
   NAT_G(2, 9, 2)
   NAT_G(2, 9, 3)
@@ -181,8 +162,8 @@ static int dispatch_general(nat_ctx* ctx, const GeneralArgs<T, XT>& a, const Col
   return NAT_ERR_UNSUPPORTED;
 }
 
-template <typename T, typename XT>
-static void fill_general(nat_ctx* ctx, GeneralArgs<T, XT>& a, const XT* xi) {
+template <typename T>
+static void fill_general(nat_ctx* ctx, GeneralArgs<T>& a) {
   const int src = ctx->cur, dst = 1 - ctx->cur;
   a.fin = (const T*)ctx->f[src];
   a.fout = (T*)ctx->f[dst];
@@ -193,19 +174,20 @@ static void fill_general(nat_ctx* ctx, GeneralArgs<T, XT>& a, const XT* xi) {
     a.N[d] = d < ctx->D ? ctx->desc.ncells[d] : 1;
   }
   a.cell = ctx->dep_cell;
+  a.xi = ctx->dep_xi64;
   a.code = ctx->dep_code32;
-  a.xi = xi;
+  a.off = ctx->dep_off32;
   double x[MAXP + 1], w[MAXP + 1];
   gll_points(ctx->p, x, w);
   for (int j = 0; j <= MAXP; j++) {
-    a.gll[j] = j <= ctx->p ? (T)x[j] : T(0);
+    a.gll[j] = j <= ctx->p ? x[j] : 0.0;
     double d = 1;
     for (int m = 0; m <= ctx->p; m++)
       if (m != j && j <= ctx->p) d *= (x[j] - x[m]);
-    a.inv_den[j] = j <= ctx->p ? (T)(1.0 / d) : T(0);
+    a.inv_den[j] = j <= ctx->p ? 1.0 / d : 0.0;
   }
   for (int j = 0; j <= MAXP; j++)
-    for (int k = 0; k <= MAXP; k++) a.dgl[j][k] = (j <= ctx->p && k <= ctx->p) ? (T)(x[j] - x[k]) : T(0);
+    for (int k = 0; k <= MAXP; k++) a.dgl[j][k] = (j <= ctx->p && k <= ctx->p) ? (x[j] - x[k]) : 0.0;
 }
 
 }  // namespace nat
@@ -215,8 +197,8 @@ int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
   const int dst = 1 - ctx->cur;
   int rc;
   if (ctx->fp32) {
-    GeneralArgs<float, float> a;
-    fill_general<float, float>(ctx, a, ctx->dep_xi32);
+    GeneralArgs<float> a;
+    fill_general<float>(ctx, a);
     CollideArgs<float> ca;
     ca.omega = (float)(1.0 / ctx->tau);
     ca.omega64 = 1.0 / ctx->tau;
@@ -226,10 +208,10 @@ int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
     ca.u = (float*)ctx->u;
     ca.mstride = ctx->n_store;
     ca.status = ctx->d_status;
-    rc = dispatch_general<float, float>(ctx, a, ca, write_macro);
+    rc = dispatch_general<float>(ctx, a, ca, write_macro);
   } else {
-    GeneralArgs<double, double> a;
-    fill_general<double, double>(ctx, a, ctx->dep_xi64);
+    GeneralArgs<double> a;
+    fill_general<double>(ctx, a);
     CollideArgs<double> ca;
     ca.omega = 1.0 / ctx->tau;
     ca.omega64 = 1.0 / ctx->tau;
@@ -239,7 +221,7 @@ int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
     ca.u = (double*)ctx->u;
     ca.mstride = ctx->n_store;
     ca.status = ctx->d_status;
-    rc = dispatch_general<double, double>(ctx, a, ca, write_macro);
+    rc = dispatch_general<double>(ctx, a, ca, write_macro);
   }
   if (rc == 0) ctx->cur = dst;
   return rc;

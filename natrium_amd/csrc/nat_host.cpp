@@ -432,7 +432,7 @@ int assemble_general(nat_ctx* ctx) {
   if (!ctx->dep_xi64) TRY(alloc(ctx, &ctx->dep_xi64, nrec * ctx->D));
   TRY(nat_launch_assemble_general(ctx, ddisp, ctx->dep_cell, ctx->dep_xi64));
   if (ctx->fp32) {
-    if (!ctx->dep_xi32) TRY(alloc(ctx, &ctx->dep_xi32, nrec * ctx->D));
+    if (!ctx->dep_off32) TRY(alloc(ctx, &ctx->dep_off32, nrec * ctx->D));
     if (!ctx->dep_code32) TRY(alloc(ctx, &ctx->dep_code32, nrec));
     const int64_t nl = ctx->D == 3 ? (int64_t)(ctx->p + 1) * (ctx->p + 1) * (ctx->p + 1) : (int64_t)(ctx->p + 1) * (ctx->p + 1);
     if ((int64_t)ctx->desc.ncells[0] * ctx->desc.ncells[1] * ctx->desc.ncells[2] * nl >= (int64_t)1 << 31)
@@ -465,7 +465,7 @@ void nat_destroy(nat_ctx* ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   void* bufs[] = {ctx->f[0],     ctx->f[1],        ctx->g[0],     ctx->g[1],       ctx->rho,       ctx->u,
                   ctx->T,        ctx->mask,        ctx->d_vert,   ctx->d_gll,      ctx->d_coords,  ctx->d_owner,
-                  ctx->d_status, ctx->d_sums,      ctx->dep_cell, ctx->dep_xi64,   ctx->dep_xi32,  ctx->dep_code32, ctx->ax_w,
+                  ctx->d_status, ctx->d_sums,      ctx->dep_cell, ctx->dep_xi64,   ctx->dep_off32, ctx->dep_code32, ctx->ax_w,
                   ctx->ax_off,   ctx->sedg_df,     ctx->sedg_geo, ctx->comp_tab};
   for (void* b : bufs)
     if (b) hipFree(b);

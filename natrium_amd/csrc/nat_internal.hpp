@@ -138,7 +138,7 @@ struct nat_ctx {
   // SL general backend: departure records per (alpha-1, node)
   int* dep_cell = nullptr;
   double* dep_xi64 = nullptr;  // D per record
-  float* dep_xi32 = nullptr;   // FP32: offset from the anchor node, D per record
+  int* dep_off32 = nullptr;    // FP32 storage: fixed-point offset from the anchor node (2^-30 units), D per record
   int* dep_code32 = nullptr;   // FP32: cell * (p+1)^D + anchor
 
   // SL structured backend: 1-D tables
