@@ -240,7 +240,10 @@ struct StructCfg {
   static constexpr int KI = (first_items() + NT - 1) / NT;
 };
 
-template <int D, int Q, int P, typename T, int TX, int TY, int TZ, int B, int WPE>
+// SYM: the host found the 1-D tables identical on all axes and mirror symmetric,
+// W(-, j, m) = W(+, (p - j) mod p, p - m) (GLL nodes are symmetric), so the p x (p+1)
+// weights of the positive shift serve every pass and stay in scalar registers.
+template <int D, int Q, int P, typename T, int TX, int TY, int TZ, int B, int WPE, bool SYM>
 __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_sl_struct(
     StructArgs<T> a, const double* __restrict__ wtab, CollideArgs<T> ca, int write_macro) {
   using Cfg = StructCfg<D, Q, P, TX, TY, TZ, B>;
@@ -270,6 +273,14 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   const unsigned ox0 = tid % TX, oy0 = (tid / TX) % TY, oz0 = tid / (TX * TY);
   const bool active = (org[0] + (int)ox0 < nx) && (org[1] + (int)oy0 < ny) && (org[2] + (int)oz0 < nz);
   const int node = (org[0] + ox0) + nx * ((org[1] + oy0) + ny * (org[2] + oz0));
+
+  double W0[SYM ? P : 1][P + 1];
+  if constexpr (SYM) {
+#pragma unroll
+    for (int j = 0; j < P; j++)
+#pragma unroll
+      for (int m = 0; m <= P; m++) W0[j][m] = wtab[j * (MAXP + 1) + m];
+  }
 
   T fs[Q];
   fs[0] = active ? a.fin[node] : T(0);
@@ -374,7 +385,11 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
         for (int m = 0; m <= P; m++) {
 #pragma unroll
           for (int kk = 0; kk < P; kk++) {
-            const double w = wt[((kk + (G.C > 0 ? 1 : 0)) % P) * (MAXP + 1) + m];
+            double w;
+            if constexpr (SYM)
+              w = (G.C > 0) ? W0[(kk + 1) % P][m] : W0[(P - kk) % P][P - m];
+            else
+              w = wt[((kk + (G.C > 0 ? 1 : 0)) % P) * (MAXP + 1) + m];
             acc[kk] = (m == 0) ? w * v[0] : acc[kk] + w * v[m];
           }
         }
@@ -457,8 +472,12 @@ static int launch_struct(nat_ctx* ctx, StructArgs<T> a, const CollideArgs<T>& ca
   a.xcd_remap = (nb % 8 == 0) ? 1 : 0;
   constexpr int B = sizeof(T) == 4 ? NAT_STRUCT_B32 : NAT_STRUCT_B64;
   constexpr int WPE = 4;  // 16 waves per CU: one 1024-thread tile, up to 128 VGPRs
-  hipLaunchKernelGGL((k_sl_struct<D, Q, P, T, TX, TY, TZ, B, WPE>), dim3(nb), dim3(TX * TY * TZ), 0, ctx->stream,
-                     a, a.w, ca, write_macro);
+  if (ctx->ax_sym)
+    hipLaunchKernelGGL((k_sl_struct<D, Q, P, T, TX, TY, TZ, B, WPE, true>), dim3(nb), dim3(TX * TY * TZ), 0,
+                       ctx->stream, a, a.w, ca, write_macro);
+  else
+    hipLaunchKernelGGL((k_sl_struct<D, Q, P, T, TX, TY, TZ, B, WPE, false>), dim3(nb), dim3(TX * TY * TZ), 0,
+                       ctx->stream, a, a.w, ca, write_macro);
   NAT_HIP(hipGetLastError());
   return 0;
 }

@@ -400,6 +400,20 @@ int setup_structured_tables(nat_ctx* ctx, bool* ok) {
                 if (std::fabs(prod) < 1e-10) return 0;
               }
   }
+  // compact tables: every axis and both signs follow from the positive-shift table of
+  // axis 0 by the GLL mirror symmetry W(a, -, j, m) = W(0, +, (p - j) mod p, p - m)
+  {
+    bool sym = true;
+    for (int a = 0; a < D && sym; a++)
+      for (int j = 0; j < p && sym; j++)
+        for (int m = 0; m <= p && sym; m++) {
+          const double wp = W(0, 0, j, m);
+          const double tol = 1e-14 * (1.0 + std::fabs(wp));
+          if (std::fabs(W(a, 0, j, m) - wp) > tol) sym = false;
+          if (std::fabs(W(a, 1, (p - j) % p, p - m) - wp) > tol) sym = false;
+        }
+    ctx->ax_sym = sym;
+  }
   // FP64 weights for both storage precisions (the kernel interpolates in FP64)
   {
     double* d;

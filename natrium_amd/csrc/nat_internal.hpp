@@ -144,6 +144,7 @@ struct nat_ctx {
   // SL structured backend: 1-D tables
   void* ax_w = nullptr;   // [3][MAXSHIFT][MAXP][MAXP+1] weights (T)
   int* ax_off = nullptr;  // [3][MAXSHIFT][MAXP] source cell offsets
+  bool ax_sym = false;    // tables equal on all axes and mirror symmetric in the shift sign
   int sid[nat::MAXQ][3] = {};
   int off_lo[3][nat::MAXSHIFT] = {}, off_hi[3][nat::MAXSHIFT] = {};
   int nshift[3] = {0, 0, 0};
