@@ -26,6 +26,7 @@ struct GeneralArgs {
   const double* xi;   //   unit coordinates, D per record
   const int* code;    // FP32 storage: cell * (p+1)^D + anchor node (-1: identity row)
   const int* off;     //   fixed-point offsets from the anchor node (2^-30 units), D per record
+  int dir_bytes;      // bytes of one direction's array (buffer-load range)
   double gll[MAXP + 1];
   double inv_den[MAXP + 1];
   double dgl[MAXP + 1][MAXP + 1];  // x_a - x_k
@@ -55,6 +56,43 @@ __device__ __forceinline__ void basis_xi(const double (&x)[MAXP + 1], const doub
     for (int m = 0; m <= P; m++)
       if (m != j) t *= (xi - x[m]);
     v[j] = t;
+  }
+}
+
+// p+1 consecutive lattice values (one x row of a source cell) through raw buffer
+// loads: byte offsets against a per-direction resource, 16-byte loads where the row
+// length allows (buffer loads need dword alignment only).
+template <int P, typename T>
+__device__ __forceinline__ void load_row(__amdgpu_buffer_rsrc_t r, int off, double (&v)[P + 1]) {
+  if constexpr (sizeof(T) == 4 && (P + 1) % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < (P + 1) / 4; q++) {
+      const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * q, 0, 0);
+#pragma unroll
+      for (int k = 0; k < 4; k++) v[4 * q + k] = (double)__builtin_bit_cast(float, (unsigned)x[k]);
+    }
+  } else if constexpr (sizeof(T) == 4 && (P + 1) % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < (P + 1) / 2; q++) {
+      const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off + 8 * q, 0, 0);
+      v[2 * q] = (double)__builtin_bit_cast(float, (unsigned)x[0]);
+      v[2 * q + 1] = (double)__builtin_bit_cast(float, (unsigned)x[1]);
+    }
+  } else if constexpr (sizeof(T) == 8 && (P + 1) % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < (P + 1) / 2; q++) {
+      const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * q, 0, 0);
+      v[2 * q] = __builtin_bit_cast(double, ((unsigned long)(unsigned)x[1] << 32) | (unsigned)x[0]);
+      v[2 * q + 1] = __builtin_bit_cast(double, ((unsigned long)(unsigned)x[3] << 32) | (unsigned)x[2]);
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m <= P; m++) {
+      if constexpr (sizeof(T) == 4)
+        v[m] = (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off + 4 * m, 0, 0));
+      else
+        v[m] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off + 8 * m, 0, 0));
+    }
   }
 }
 
@@ -103,7 +141,43 @@ __global__ __launch_bounds__(256) void k_sl_general(GeneralArgs<T> a, CollideArg
     const int cz = (D == 3) ? c / (a.N[0] * a.N[1]) : 0;
     const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P + (int64_t)n1 * ((int64_t)cz * P));
     double acc = 0;
-    if constexpr (D == 2) {
+    // When no product weight can fall under the reference cut-off (product of the
+    // smallest 1-D magnitudes >= 1e-10) the sum factorises row by row; otherwise every
+    // product is formed and tested as in the assembled operator.
+    double mn = 1;
+    {
+      double mx_ = fabs(lx[0]), my_ = fabs(ly[0]), mz_ = (D == 3) ? fabs(lz[0]) : 1.0;
+#pragma unroll
+      for (int m = 1; m <= P; m++) {
+        mx_ = fmin(mx_, fabs(lx[m]));
+        my_ = fmin(my_, fabs(ly[m]));
+        if constexpr (D == 3) mz_ = fmin(mz_, fabs(lz[m]));
+      }
+      mn = mx_ * my_ * mz_;
+    }
+    if (mn >= 1e-10) {
+      const uint64_t pb = (uint64_t)src;
+      const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pb >> 32));
+      const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pb);
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(((uint64_t)phi << 32) | (uint64_t)plo), (short)0, __builtin_amdgcn_readfirstlane(a.dir_bytes),
+          0x00020000);
+      const int b0 = (int)base * (int)sizeof(T);
+#pragma unroll
+      for (int mz = 0; mz < ((D == 3) ? NP : 1); mz++) {
+        double accz = 0;
+#pragma unroll
+        for (int my = 0; my <= P; my++) {
+          double v[NP];
+          load_row<P, T>(rs, b0 + (int)sizeof(T) * (n0 * (my + n1 * mz)), v);
+          double row = lx[0] * v[0];
+#pragma unroll
+          for (int mx = 1; mx <= P; mx++) row += lx[mx] * v[mx];
+          accz += ly[my] * row;
+        }
+        acc += ((D == 3) ? lz[mz] : 1.0) * accz;
+      }
+    } else if constexpr (D == 2) {
 #pragma unroll
       for (int my = 0; my <= P; my++)
 #pragma unroll
@@ -177,6 +251,7 @@ static void fill_general(nat_ctx* ctx, GeneralArgs<T>& a) {
   a.xi = ctx->dep_xi64;
   a.code = ctx->dep_code32;
   a.off = ctx->dep_off32;
+  a.dir_bytes = (int)(ctx->stride * (int64_t)sizeof(T));
   double x[MAXP + 1], w[MAXP + 1];
   gll_points(ctx->p, x, w);
   for (int j = 0; j <= MAXP; j++) {
@@ -194,6 +269,10 @@ static void fill_general(nat_ctx* ctx, GeneralArgs<T>& a) {
 
 int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
   using namespace nat;
+  if (ctx->stride * (int64_t)(ctx->fp32 ? 4 : 8) > INT32_MAX) {
+    ctx->err = "general backend: one direction's array exceeds 2 GiB";
+    return NAT_ERR_UNSUPPORTED;
+  }
   const int dst = 1 - ctx->cur;
   int rc;
   if (ctx->fp32) {
