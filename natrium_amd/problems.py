@@ -118,3 +118,20 @@ def divergence_free_field(x, modes=8, amplitude=0.05, seed=1234, L=2 * math.pi):
         u += np.outer(a, s)
     nrm = np.sqrt((u ** 2).sum(0).mean())
     return u * (amplitude / max(nrm, 1e-300))
+
+
+def couette2d_velocity(y, t, U, L=1.0, viscosity=1.0):
+    """CouetteFlow2D::AnalyticVelocity (benchmarks/CouetteFlow2D.cpp:87-125): x-velocity of the
+    unsteady plane Couette flow on [0, L]^2 (bottom wall at rest, top wall moving with U,
+    started from rest at t = 0), as its sine series truncated where exp(-nu lambda^2 t) < 1e-20."""
+    y = np.asarray(y, dtype=np.float64)
+    if t < 0.00001:
+        return np.where(y < L - 0.00001, 0.0, U)
+    s = U * y / L
+    for i in range(1, 10001):
+        lam = i * math.pi / L
+        e = math.exp(-viscosity * lam * lam * t)
+        s = s + (1.0 if i % 2 == 0 else -1.0) * 2 * U / (lam * L) * e * np.sin(lam * y)
+        if e < 1e-20:
+            break
+    return s

@@ -68,6 +68,12 @@ def lib():
         L.orc_rk5_scalar.restype = ctypes.c_double
         L.orc_rk5_scalar.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_set_compressible_options.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.orc_set_walls.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_op_hits.argtypes = [ctypes.c_void_p, _lp, _lp]
+        L.orc_op_src.restype = ctypes.c_int
+        L.orc_op_src.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int8)]
+        L.orc_op_bvec.restype = ctypes.c_int
+        L.orc_op_bvec.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -128,7 +134,11 @@ class Oracle:
     """CFDSolver-like FP64 CPU solver (SL or SEDG, incompressible or compressible)."""
 
     def __init__(self, stencil, p, ncells, vertices, cartesian=True, periodic=None, scaling=1.0, viscosity=1.0,
-                 cfl=0.4, scheme="SL", compressible=False, gamma=1.4, prandtl=0.0, sutherland=False):
+                 cfl=0.4, scheme="SL", compressible=False, gamma=1.4, prandtl=0.0, sutherland=False,
+                 wall_velocity=None):
+        """periodic: flag per axis (default all periodic). Faces of non-periodic axes are
+        velocity-neq bounce-back walls; wall_velocity maps a box face 2*axis+side to its
+        velocity (default at rest)."""
         sid, D, Q = STENCILS[stencil]
         self.D, self.Q, self.p = D, Q, p
         nc = np.array(list(ncells) + [1] * (3 - len(ncells)), dtype=np.int32)
@@ -143,6 +153,10 @@ class Oracle:
         self.compressible = compressible
         if compressible:
             lib().orc_set_compressible_options(self.h, float(prandtl), int(sutherland))
+        wu = np.zeros((6, 3))
+        for f, v in (wall_velocity or {}).items():
+            wu[int(f), :len(v)] = v
+        lib().orc_set_walls(self.h, _p(wu))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -184,6 +198,26 @@ class Oracle:
 
     def identity_rows(self):
         return lib().orc_op_identity_rows(self.h)
+
+    def operator_src(self, alpha):
+        """Source direction of every entry of block alpha (None: all read alpha)."""
+        nnz = lib().orc_op_nnz(self.h, alpha)
+        src = np.zeros(nnz, dtype=np.int8)
+        if lib().orc_op_src(self.h, alpha, _p(src, ctypes.POINTER(ctypes.c_int8))):
+            return src
+        return None
+
+    def boundary_values(self, alpha):
+        """Values added to streamed f_alpha at wall hits (None: no hit)."""
+        b = np.zeros(self.n)
+        return b if lib().orc_op_bvec(self.h, alpha, _p(b)) else None
+
+    def hits(self):
+        """(hits, cells holding hits) of the boundary handler's hit list."""
+        h = np.zeros(1, dtype=np.int64)
+        c = np.zeros(1, dtype=np.int64)
+        lib().orc_op_hits(self.h, _p(h, _lp), _p(c, _lp))
+        return int(h[0]), int(c[0])
 
     def locate(self, dof, alpha):
         xi = np.zeros(3)

@@ -140,6 +140,35 @@ int orc_assemble(orc_solver* s, double dt) {
 
 int64_t orc_op_nnz(orc_solver* s, int alpha) { return s->op.block[alpha - 1].rowptr[s->n]; }
 int64_t orc_op_identity_rows(orc_solver* s) { return s->op.n_identity_rows; }
+
+// Velocity of the velocity-neq bounce-back wall on each box face (6 x 3, face
+// 2*axis+side), used on non-periodic axes; set before orc_assemble.
+void orc_set_walls(orc_solver* s, const double* wall_u) {
+  for (int f = 0; f < 6; f++)
+    for (int d = 0; d < 3; d++) s->mesh.wall_u[f][d] = wall_u[3 * f + d];
+  if (s->scheme == 0) s->feq.mesh = s->mesh;
+}
+
+// Boundary hit list counts of the last assembly (SemiLagrangianBoundaryHandler::n_hits, n_cells)
+void orc_op_hits(orc_solver* s, int64_t* n_hits, int64_t* n_cells) {
+  *n_hits = s->op.n_hits;
+  *n_cells = s->op.n_hit_cells;
+}
+
+// Source direction of every entry of block alpha (returns 0 and writes nothing when
+// all entries read direction alpha), and the boundary values added after the product.
+int orc_op_src(orc_solver* s, int alpha, int8_t* src) {
+  const CSR& A = s->op.block[alpha - 1];
+  if (A.src.empty()) return 0;
+  std::memcpy(src, A.src.data(), A.src.size());
+  return 1;
+}
+int orc_op_bvec(orc_solver* s, int alpha, double* b) {
+  const std::vector<double>& v = s->op.bvec[alpha - 1];
+  if (v.empty()) return 0;
+  std::memcpy(b, v.data(), sizeof(double) * v.size());
+  return 1;
+}
 void orc_op_get(orc_solver* s, int alpha, int64_t* rowptr, int64_t* col, double* val) {
   const CSR& A = s->op.block[alpha - 1];
   std::memcpy(rowptr, A.rowptr.data(), sizeof(int64_t) * (s->n + 1));
@@ -192,12 +221,18 @@ int orc_collide(orc_solver* s) {
   return e;
 }
 
-static void stream_vec(orc_solver* s, std::vector<double>& v) {
+// One SL / SEDG streaming of a population vector. For SL, with_boundary adds the
+// velocity-neq bounce-back values (applyBoundaryConditions after the product,
+// CFDSolver.cpp:669-721); the g populations of the compressible solver get the
+// product only, since applyBoundaryConditionsToG acts on velocity-equilibrium
+// boundaries alone (CompressibleCFDSolver.h:279-314,
+// SemiLagrangianBoundaryHandler.cpp:111-158).
+static void stream_vec(orc_solver* s, std::vector<double>& v, bool with_boundary) {
   const int Q = s->st.Q;
   const int64_t n = s->n;
   if (s->scheme == 0) {
     s->ftmp = v;  // DistributionFunctions f_tmp(m_f)
-    for (int a = 1; a < Q; a++) csr_vmult(s->op.block[a - 1], s->ftmp.data() + (size_t)a * n, v.data() + (size_t)a * n, n);
+    sl_apply(s->op, s->ftmp.data(), v.data(), with_boundary);
   } else {
     std::vector<double*> fp(Q - 1);
     for (int a = 1; a < Q; a++) fp[a - 1] = v.data() + (size_t)a * n;
@@ -206,8 +241,8 @@ static void stream_vec(orc_solver* s, std::vector<double>& v) {
 }
 
 int orc_stream(orc_solver* s) {
-  stream_vec(s, s->f);
-  if (s->compressible) stream_vec(s, s->g);
+  stream_vec(s, s->f, true);
+  if (s->compressible) stream_vec(s, s->g, false);
   s->time += s->dt;
   return 0;
 }

@@ -54,6 +54,9 @@ struct Mesh {
   int D = 2;
   int N[3] = {1, 1, 1};
   bool periodic[3] = {true, true, true};
+  // velocity of the velocity-neq bounce-back wall on box face 2*axis+side, used on
+  // non-periodic axes (boundary ids of a colorized hyper-rectangle)
+  double wall_u[6][3] = {};
   bool cartesian = true;  // MappingCartesian if true, else MappingQ1
   std::vector<std::array<double, 3>> vert;  // (N0+1)(N1+1)(N2+1) vertices
 

@@ -2,6 +2,7 @@
 #include "orc_sl.hpp"
 
 #include <algorithm>
+#include <set>
 #include <cmath>
 #include <stdexcept>
 
@@ -182,12 +183,15 @@ int face_crossed_first(const Mesh& mesh, int cell, const double pin[3], const do
   return face;
 }
 
-int locate_departure(const FEQSpace& V, const Stencil& st, double dt, int64_t dof, int alpha, double xi[3]) {
+int locate_departure(const FEQSpace& V, const Stencil& st, double dt, int64_t dof, int alpha, double xi[3],
+                     int* src_dir, std::vector<WallHit>* hits) {
   const Mesh& mesh = V.mesh;
   double cur[3] = {V.x[dof][0], V.x[dof][1], V.x[dof][2]};
   double dep[3];
   for (int d = 0; d < 3; d++) dep[d] = cur[d] + st.dir[alpha][d] * (-dt);
   int cell = V.owner[dof];
+  int dir = alpha;
+  if (src_dir) *src_dir = alpha;
   int life = 0;
   while (true) {
     life++;
@@ -202,7 +206,25 @@ int locate_departure(const FEQSpace& V, const Stencil& st, double dt, int64_t do
     if (face == -1) break;
     bool wrapped;
     int nb = neighbor_cell(mesh, cell, face, &wrapped);
-    if (nb < 0) throw std::runtime_error("non-periodic boundary reached: wall boundaries are not part of this oracle");
+    if (nb < 0) {
+      // Velocity-neq bounce-back wall (SemiLagrangian.cpp:358-385): the path turns
+      // into the opposite direction at the hit point p_b and continues for the
+      // remaining distance; the cell does not change.
+      dir = st.opp[dir];
+      double vel = 0, dist = 0;
+      for (int d = 0; d < mesh.D; d++) {
+        vel += st.dir[dir][d] * st.dir[dir][d];
+        dist += (dep[d] - pb[d]) * (dep[d] - pb[d]);
+      }
+      vel = std::sqrt(vel);
+      dist = std::sqrt(dist);
+      for (int d = 0; d < mesh.D; d++) {
+        cur[d] = pb[d];
+        dep[d] = pb[d] - st.dir[dir][d] * dist / vel;
+      }
+      if (hits) hits->push_back({face, cell});
+      continue;
+    }
     if (wrapped) {
       double c2[3], d2[3];
       across_periodic(mesh, cell, face, pb, c2);
@@ -216,6 +238,7 @@ int locate_departure(const FEQSpace& V, const Stencil& st, double dt, int64_t do
     }
     cell = nb;
   }
+  if (src_dir) *src_dir = dir;
   if (!mesh.inverse_map(cell, dep, xi)) throw std::runtime_error("inverse map failed at departure point");
   for (int d = 0; d < mesh.D; d++) {
     clamp_unit(xi[d]);
@@ -232,17 +255,40 @@ void assemble_sl(const FEQSpace& V, const Stencil& st, double dt, SLOperator& op
   op.Q = st.Q;
   op.n = n;
   op.block.assign(st.Q - 1, CSR());
+  op.bvec.assign(st.Q - 1, std::vector<double>());
+  op.n_hits = 0;
+  std::set<int> hit_cells;
   int64_t n_ident = 0;
   for (int alpha = 1; alpha < st.Q; alpha++) {
     CSR& A = op.block[alpha - 1];
-    std::vector<int> cellv(n);
+    std::vector<int> cellv(n), srcv(n);
     std::vector<std::array<double, 3>> xiv(n);
+    std::vector<std::vector<WallHit>> hitv(n);
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : n_ident)
     for (int64_t i = 0; i < n; i++) {
       double xi[3];
-      cellv[i] = locate_departure(V, st, dt, i, alpha, xi);
+      cellv[i] = locate_departure(V, st, dt, i, alpha, xi, &srcv[i], &hitv[i]);
       xiv[i] = {xi[0], xi[1], xi[2]};
       if (cellv[i] < 0) n_ident++;
+    }
+    // VelocityNeqBounceBack::calculateBoundaryValues: per hit, 2 w_alpha rho (e_alpha . u_w) / cs2
+    // with rho = 1 and the destination direction alpha (scaled e and cs2)
+    bool any_hit = false, any_flip = false;
+    for (int64_t i = 0; i < n; i++) {
+      if (!hitv[i].empty()) any_hit = true;
+      if (cellv[i] >= 0 && srcv[i] != alpha) any_flip = true;
+    }
+    if (any_hit) {
+      std::vector<double>& b = op.bvec[alpha - 1];
+      b.assign(n, 0.0);
+      for (int64_t i = 0; i < n; i++)
+        for (const WallHit& h : hitv[i]) {
+          double exu = 0;
+          for (int d = 0; d < st.D; d++) exu += V.mesh.wall_u[h.face][d] * st.dir[alpha][d];
+          b[i] = b[i] + 2 * st.w[alpha] * 1.0 * exu / st.cs2;
+          op.n_hits++;
+          hit_cells.insert(h.cell);
+        }
     }
     A.rowptr.assign(n + 1, 0);
     std::vector<int> cnt(n, 0);
@@ -263,10 +309,12 @@ void assemble_sl(const FEQSpace& V, const Stencil& st, double dt, SLOperator& op
     for (int64_t i = 0; i < n; i++) A.rowptr[i + 1] = A.rowptr[i] + cnt[i];
     A.col.resize(A.rowptr[n]);
     A.val.resize(A.rowptr[n]);
+    if (any_flip) A.src.assign(A.rowptr[n], (int8_t)alpha);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
       int64_t o = A.rowptr[i];
       if (cellv[i] < 0) {
+        // life-time limit: 1 on the diagonal of block (alpha, alpha)
         A.col[o] = i;
         A.val[o] = 1.0;
         continue;
@@ -285,11 +333,33 @@ void assemble_sl(const FEQSpace& V, const Stencil& st, double dt, SLOperator& op
       for (auto& pr : e) {
         A.col[o] = pr.first;
         A.val[o] = pr.second;
+        if (any_flip) A.src[o] = (int8_t)srcv[i];
         o++;
       }
     }
   }
   op.n_identity_rows = n_ident;
+  op.n_hit_cells = (int64_t)hit_cells.size();
+}
+
+void sl_apply(const SLOperator& op, const double* x, double* y, bool with_boundary) {
+  const int64_t n = op.n;
+  for (int a = 1; a < op.Q; a++) {
+    const CSR& A = op.block[a - 1];
+    const std::vector<double>& b = op.bvec[a - 1];
+    const bool flip = !A.src.empty();
+    double* ya = y + (size_t)a * n;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+      double s = 0;
+      for (int64_t k = A.rowptr[i]; k < A.rowptr[i + 1]; k++) {
+        const int src = flip ? A.src[k] : a;
+        s += A.val[k] * x[(size_t)src * n + A.col[k]];
+      }
+      if (with_boundary && !b.empty()) s = s + b[i];
+      ya[i] = s;
+    }
+  }
 }
 
 void csr_vmult(const CSR& A, const double* x, double* y, int64_t n) {

@@ -4,7 +4,10 @@
 // Reference: advection/SemiLagrangian.cpp:145-522 (fillSparseObject),
 //            advection/SemiLagrangian.cpp:525-757 (faceCrossedFirst[WithoutMapping]),
 //            advection/SemiLagrangianTools.cpp:15-64 (shapeFunctionValue),
-//            boundaries/PeriodicBoundary.cpp:184-310 (coordinatesAcrossPeriodicBoundary).
+//            boundaries/PeriodicBoundary.cpp:184-310 (coordinatesAcrossPeriodicBoundary),
+//            advection/SemiLagrangian.cpp:358-385 (velocity-neq bounce-back: reverse the path),
+//            boundaries/VelocityNeqBounceBack.cpp:137-195 (value added per wall hit),
+//            boundaries/SemiLagrangianBoundaryHandler.cpp:14-158 (hit list; g untouched).
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -35,17 +38,31 @@ struct FEQSpace {
   void shape_values(const double xi[3], double* out) const;
 };
 
-// Block-diagonal SL operator: one CSR matrix per streamed direction alpha=1..Q-1.
+// SL operator: one CSR matrix per streamed destination direction alpha=1..Q-1.
+// Columns are DoFs; `src` holds the source direction of each entry when a wall
+// couples alpha with its opposite (block (alpha, opp alpha) of the reference's
+// block matrix), and is empty when every entry reads direction alpha.
 struct CSR {
   std::vector<int64_t> rowptr;
   std::vector<int64_t> col;
   std::vector<double> val;
+  std::vector<int8_t> src;
 };
 struct SLOperator {
   int Q = 0;
   int64_t n = 0;
   std::vector<CSR> block;  // block[alpha-1]
   int64_t n_identity_rows = 0;  // rows written by the tracker life-time limit
+  // Velocity-neq bounce-back: per destination direction, the values added to the
+  // streamed f after the product (empty when no path reaches a wall), and the
+  // hit-list counts of the boundary handler (hits, distinct cells holding hits).
+  std::vector<std::vector<double>> bvec;
+  int64_t n_hits = 0, n_hit_cells = 0;
+};
+
+struct WallHit {
+  int face;  // box face 2*axis+side
+  int cell;  // cell in which the hit lies
 };
 
 // Returns the crossed face id or -1 (point inside). Throws on inverse-map failure
@@ -62,12 +79,19 @@ void across_periodic(const Mesh& mesh, int cell, int f, const double p[3], doubl
 
 // Locate the departure point of DoF `dof` for direction alpha (walk from its
 // owning cell). Returns the final cell (or -1 when the life-time limit hit) and
-// the unit coordinates there.
-int locate_departure(const FEQSpace& V, const Stencil& st, double dt, int64_t dof, int alpha, double xi[3]);
+// the unit coordinates there; *src_dir receives the direction the path ends in
+// (alpha, or its opposite after an odd number of wall reflections) and `hits`
+// the walls met on the way.
+int locate_departure(const FEQSpace& V, const Stencil& st, double dt, int64_t dof, int alpha, double xi[3],
+                     int* src_dir = nullptr, std::vector<WallHit>* hits = nullptr);
 
 void assemble_sl(const FEQSpace& V, const Stencil& st, double dt, SLOperator& op);
 
-// y = A_alpha x for one block
+// y = A_alpha x for one block whose entries all read x (no wall coupling)
 void csr_vmult(const CSR& A, const double* x, double* y, int64_t n);
+
+// Streaming of all directions: y_alpha = sum_entries val x_{src}[col] (+ bvec_alpha
+// when with_boundary), x and y laid out direction-major (Q x n); y_0 is not touched.
+void sl_apply(const SLOperator& op, const double* x, double* y, bool with_boundary);
 
 }  // namespace orc
