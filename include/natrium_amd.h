@@ -56,7 +56,8 @@ typedef struct nat_desc {
   int ncells[3];         /* cells per axis */
   const double* vertices;/* lattice vertices, see conventions above */
   int cartesian;         /* 1: axis-aligned cells (MappingCartesian); 0: trilinear map */
-  int periodic[3];       /* periodic flag per axis (walls are not part of this build) */
+  int periodic[3];       /* periodic flag per axis; both faces of a non-periodic axis are
+                            velocity-neq bounce-back walls (SL scheme only) */
   double scaling;        /* stencil scaling s (|c_axis| = s) */
   double viscosity;      /* kinematic viscosity nu */
   double cfl;            /* CFL number; dt = CFL h_min / (sqrt(2) s p^2) */
@@ -69,6 +70,11 @@ typedef struct nat_desc {
   int backend;           /* NAT_BACKEND_* (SL only) */
   double prandtl;        /* compressible: Prandtl number (0 = not set, unit Prandtl) */
   int sutherland;        /* compressible: 1 = Sutherland viscosity law (T/0.85)^0.7 */
+  double wall_velocity[6][3]; /* velocity of the wall on box face 2*axis+side (side 0: lower),
+                                 constant in time; read on non-periodic axes. Replaces
+                                 VelocityNeqBounceBack(boundary id, velocity)
+                                 (boundaries/VelocityNeqBounceBack.cpp:15-38) with the boundary
+                                 ids of a colorized box */
 } nat_desc;
 
 /* Context lifetime. Replaces the CFDSolver<dim> constructor's set-up of the

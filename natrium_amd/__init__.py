@@ -39,6 +39,7 @@ class _Desc(ctypes.Structure):
         ("precision", ctypes.c_int), ("equilibrium", ctypes.c_int), ("compressible", ctypes.c_int),
         ("gamma", ctypes.c_double), ("rho0", ctypes.c_double), ("backend", ctypes.c_int),
         ("prandtl", ctypes.c_double), ("sutherland", ctypes.c_int),
+        ("wall_velocity", (ctypes.c_double * 3) * 6),
     ]
 
 
@@ -113,7 +114,11 @@ class Solver:
 
     def __init__(self, stencil, order, ncells, vertices, cartesian=True, periodic=None, scaling=1.0,
                  viscosity=1.0, cfl=0.4, scheme="SL", precision="fp64", backend="auto", rho0=1.0,
-                 compressible=False, gamma=1.4, equilibrium=None, prandtl=0.0, sutherland=False):
+                 compressible=False, gamma=1.4, equilibrium=None, prandtl=0.0, sutherland=False,
+                 wall_velocity=None):
+        """periodic: flag per axis (default: all periodic). Both faces of a non-periodic axis
+        are velocity-neq bounce-back walls (SL only); wall_velocity maps a box face
+        2*axis+side (side 0 = lower) to its constant velocity (default: at rest)."""
         L = lib()
         self.stencil = stencil
         self.D = 3 if stencil in ("D3Q19", "D3Q27") else 2
@@ -142,6 +147,9 @@ class Solver:
         d.backend = BACKENDS[backend]
         d.prandtl = prandtl
         d.sutherland = int(sutherland)
+        for f, v in (wall_velocity or {}).items():
+            for k, x in enumerate(v):
+                d.wall_velocity[int(f)][k] = float(x)
         h = ctypes.c_void_p()
         rc = L.nat_create(ctypes.byref(d), ctypes.byref(h))
         self._h = h

@@ -4,6 +4,8 @@
 // Follows the reference semi-Lagrangian assembly:
 //   advection/SemiLagrangian.cpp:196-245  departure point per (DoF, direction), start cell
 //   advection/SemiLagrangian.cpp:246-443  path walk: life-time limit 50, periodic wrap, interior faces
+//   advection/SemiLagrangian.cpp:358-385  velocity-neq bounce-back: reverse the path at a wall
+//   boundaries/VelocityNeqBounceBack.cpp:137-195 value added per wall hit
 //   advection/SemiLagrangian.cpp:525-699  faceCrossedFirst: unit-cell clamps (1e-10), x<y<z tie order
 //   advection/SemiLagrangian.cpp:702-757  faceCrossedFirstWithoutMapping (inverse-map failure)
 //   advection/SemiLagrangianTools.cpp:39-52 unit point of the departure point, clamped
@@ -209,14 +211,19 @@ __device__ int crossed_first(const MeshDev& m, const int c[3], const double pin[
 }
 
 // Walk from the owning cell to the cell holding x - dt c. Returns the cell index,
-// -1 when the life-time limit is hit (identity row), -2 at a wall, -3 when the
-// final unit point is outside its cell.
-__device__ int locate(const MeshDev& m, int owner, const double x[3], const double disp[3], double xi[3]) {
+// -1 when the life-time limit is hit (identity row), -3 when the final unit point
+// is outside its cell. At a wall face the path turns into the opposite direction
+// at the hit point and continues for the remaining distance in the same cell;
+// *nhit counts the wall hits (an odd count reads the opposite direction) and *bias
+// accumulates 2 w (e . u_wall) / cs2 (rho = 1, destination direction e) per hit.
+__device__ int locate(const MeshDev& m, int owner, const double x[3], const double disp[3], const double e[3],
+                      double w, double cs2, double xi[3], int* nhit, double* bias) {
   double cur[3] = {x[0], x[1], x[2]};
   double dep[3] = {x[0] + disp[0], x[1] + disp[1], x[2] + disp[2]};
   int c[3];
   cell_xyz(m, owner, c);
   int cell = owner;
+  double sgn = 1.0;  // current direction = sgn * e
   for (int life = 1;; life++) {
     if (life > 50) return -1;
     double pb[3];
@@ -225,7 +232,26 @@ __device__ int locate(const MeshDev& m, int owner, const double x[3], const doub
     bool wrapped;
     int nc[3];
     const int nb = neighbor(m, c, face, &wrapped, nc);
-    if (nb < 0) return -2;
+    if (nb < 0) {
+      sgn = -sgn;
+      double vel = 0, dist = 0;
+      for (int d = 0; d < m.D; d++) {
+        const double ed = sgn * e[d];
+        vel += ed * ed;
+        dist += (dep[d] - pb[d]) * (dep[d] - pb[d]);
+      }
+      vel = sqrt(vel);
+      dist = sqrt(dist);
+      for (int d = 0; d < m.D; d++) {
+        cur[d] = pb[d];
+        dep[d] = pb[d] - (sgn * e[d]) * dist / vel;
+      }
+      double exu = 0;
+      for (int d = 0; d < m.D; d++) exu += m.wall_u[face][d] * e[d];
+      *bias = *bias + 2 * w * 1.0 * exu / cs2;
+      (*nhit)++;
+      continue;
+    }
     if (wrapped) {
       periodic_shift(m, c, face, pb);
       periodic_shift(m, c, face, dep);
@@ -245,23 +271,52 @@ __device__ int locate(const MeshDev& m, int owner, const double x[3], const doub
   return cell;
 }
 
-// One thread per (streamed direction, DoF).
+// One thread per (streamed direction, DoF). dirw holds per streamed direction the
+// scaled velocity (3) and the weight; with walls, flipmask[i] collects bit alpha
+// for a reversed source direction and bit 0 for any hit, bias_out the added value.
 __global__ void k_assemble_general(MeshDev m, int64_t n, int Qm1, const double* __restrict__ coords,
                                    const int* __restrict__ owner, const double* __restrict__ disp /* Qm1 x 3 */,
-                                   int* __restrict__ cell_out, double* __restrict__ xi_out, int* __restrict__ status) {
+                                   const double* __restrict__ dirw /* Qm1 x 4 */, double cs2,
+                                   int* __restrict__ cell_out, double* __restrict__ xi_out,
+                                   unsigned* __restrict__ flipmask, double* __restrict__ bias_out,
+                                   int* __restrict__ status) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * Qm1) return;
   const int a = (int)(t / n);
   const int64_t i = t - (int64_t)a * n;
   const double x[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
   const double dp[3] = {disp[3 * a], disp[3 * a + 1], disp[3 * a + 2]};
+  const double e[3] = {dirw[4 * a], dirw[4 * a + 1], dirw[4 * a + 2]};
   double xi[3] = {0, 0, 0};
-  const int c = locate(m, owner[i], x, dp, xi);
+  int nhit = 0;
+  double bias = 0;
+  const int c = locate(m, owner[i], x, dp, e, dirw[4 * a + 3], cs2, xi, &nhit, &bias);
   cell_out[t] = c;
   for (int d = 0; d < m.D; d++) xi_out[t * m.D + d] = xi[d];
-  if (c == -2) atomicOr(status, 2);
+  if (flipmask) {
+    bias_out[t] = bias;
+    // an identity row (life-time limit) keeps direction alpha; its hits still add
+    const bool flip = (nhit & 1) && c >= 0;
+    const unsigned bits = (flip ? (1u << (a + 1)) : 0u) | (nhit ? 1u : 0u);
+    if (bits) atomicOr(&flipmask[i], bits);
+  } else if (nhit) {
+    atomicOr(status, 2);
+  }
   if (c == -3) atomicOr(status, 4);
   if (c == -1) atomicOr(status, 8);
+}
+
+// Compact wall table: for the k-th wall node list[k], its flip mask and the
+// per-direction added values (Q per entry, direction 0 unused).
+__global__ void k_wall_compact(int64_t n, int Q, const int* __restrict__ list, int64_t nwall,
+                               const unsigned* __restrict__ flipmask, const double* __restrict__ bias_rec,
+                               unsigned* __restrict__ wall_flip, double* __restrict__ wall_bias) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nwall) return;
+  const int64_t i = list[k];
+  wall_flip[k] = flipmask[i];
+  wall_bias[k * Q] = 0;
+  for (int a = 1; a < Q; a++) wall_bias[k * Q + a] = bias_rec[(int64_t)(a - 1) * n + i];
 }
 
 // Lagrange basis on GLL nodes: value of basis j at xi (product form).
@@ -324,19 +379,32 @@ __global__ void k_assemble_axis(int D, const int* N, const double* vert_axis /* 
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-int nat_launch_assemble_general(nat_ctx* ctx, const double* d_disp, int* d_cell, double* d_xi) {
+int nat_launch_assemble_general(nat_ctx* ctx, const double* d_disp, const double* d_dirw, int* d_cell, double* d_xi,
+                                unsigned* d_flipmask, double* d_bias) {
   nat::MeshDev m;
   m.D = ctx->D;
   for (int a = 0; a < 3; a++) {
     m.N[a] = a < ctx->D ? ctx->desc.ncells[a] : 1;
     m.periodic[a] = a < ctx->D ? ctx->desc.periodic[a] : 1;
   }
+  for (int f = 0; f < 6; f++)
+    for (int d = 0; d < 3; d++) m.wall_u[f][d] = d < ctx->D ? ctx->desc.wall_velocity[f][d] : 0.0;
   m.cartesian = ctx->desc.cartesian;
   m.vert = ctx->d_vert;
   const int64_t total = ctx->n_dofs * (ctx->Q - 1);
   const int bs = 128;
   hipLaunchKernelGGL(nat::k_assemble_general, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, ctx->stream, m,
-                     ctx->n_dofs, ctx->Q - 1, ctx->d_coords, ctx->d_owner, d_disp, d_cell, d_xi, ctx->d_status);
+                     ctx->n_dofs, ctx->Q - 1, ctx->d_coords, ctx->d_owner, d_disp, d_dirw, ctx->st.cs2, d_cell, d_xi,
+                     d_flipmask, d_bias, ctx->d_status);
+  NAT_HIP(hipGetLastError());
+  return 0;
+}
+
+int nat_launch_wall_compact(nat_ctx* ctx, const int* d_list, const unsigned* d_flipmask, const double* d_bias) {
+  if (ctx->n_wall == 0) return 0;
+  const int bs = 256;
+  hipLaunchKernelGGL(nat::k_wall_compact, dim3((unsigned)((ctx->n_wall + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
+                     ctx->n_dofs, ctx->Q, d_list, ctx->n_wall, d_flipmask, d_bias, ctx->wall_flip, ctx->wall_bias);
   NAT_HIP(hipGetLastError());
   return 0;
 }

@@ -36,6 +36,10 @@ struct CompArgs {
   int N[3];
   const int* cell;    // [(alpha-1) * n + i] departure cell (-1: identity row)
   const double* xi;   // unit coordinates, 2 per record
+  const int* wall_idx;        // walls: wall entry per DoF or -1 (null: no walls)
+  const unsigned* wall_flip;  //   per entry: bit alpha set when alpha reads the opposite direction
+  const double* wall_bias;    //   per entry: Q values added to f after interpolation
+  int opp[MAXQ];              // opposite direction index
   double gll[MAXP + 1];
   double inv_den[MAXP + 1];
   double cs2;         // unscaled c_s^2
@@ -196,16 +200,27 @@ __global__ __launch_bounds__(256) void k_sl_comp(CompArgs<T> a, const double* __
   fl[0] = (double)a.fin[i] + tab[CT_W] * a.rho0;
   gl[0] = (double)a.gin[i] + tab[CT_W] * a.gshift;
   const int n0 = a.nfe[0];
+  int widx = -1;
+  unsigned wmask = 0;
+  if (a.wall_idx) {
+    widx = a.wall_idx[i];
+    if (widx >= 0) wmask = a.wall_flip[widx];
+  }
 #pragma unroll
   for (int al = 1; al < Q; al++) {
     const int64_t r = (int64_t)(al - 1) * a.n + i;
-    const T* fsrc = a.fin + (int64_t)al * a.stride;
-    const T* gsrc = a.gin + (int64_t)al * a.stride;
+    // velocity-neq bounce-back: read the opposite direction at the reflected point; the
+    // wall value is added to f only (g is streamed by the same matrix with no boundary
+    // values, CompressibleCFDSolver.h:279-314)
+    const int sd = ((wmask >> al) & 1u) ? a.opp[al] : al;
+    const T* fsrc = a.fin + (int64_t)sd * a.stride;
+    const T* gsrc = a.gin + (int64_t)sd * a.stride;
+    const double bias = widx >= 0 ? a.wall_bias[(int64_t)widx * Q + al] : 0.0;
     const int c = a.cell[r];
     double af, ag;
     if (c < 0) {
-      af = (double)fsrc[i];
-      ag = (double)gsrc[i];
+      af = (double)a.fin[(int64_t)al * a.stride + i];
+      ag = (double)a.gin[(int64_t)al * a.stride + i];
     } else {
       double lx[P + 1], ly[P + 1];
       lag_all<P>(a.gll, a.inv_den, a.xi[2 * r + 0], lx);
@@ -226,6 +241,7 @@ __global__ __launch_bounds__(256) void k_sl_comp(CompArgs<T> a, const double* __
           }
         }
     }
+    af = af + bias;
     // shifted storage: interpolation preserves constants, so the shift is added back here
     fl[al] = af + tab[al * CTAB + CT_W] * a.rho0;
     gl[al] = ag + tab[al * CTAB + CT_W] * a.gshift;
@@ -271,6 +287,15 @@ static void fill_comp(nat_ctx* ctx, CompArgs<T>& a, int src, int dst) {
   }
   a.cell = ctx->dep_cell;
   a.xi = ctx->dep_xi64;
+  a.wall_idx = ctx->has_walls ? ctx->wall_idx : nullptr;
+  a.wall_flip = ctx->wall_flip;
+  a.wall_bias = ctx->wall_bias;
+  for (int q = 0; q < MAXQ; q++) a.opp[q] = q;
+  for (int q = 0; q < ctx->Q; q++)
+    for (int r = 0; r < ctx->Q; r++)
+      if (ctx->st.dir[r][0] == -ctx->st.dir[q][0] && ctx->st.dir[r][1] == -ctx->st.dir[q][1] &&
+          ctx->st.dir[r][2] == -ctx->st.dir[q][2])
+        a.opp[q] = r;
   double x[MAXP + 1], w[MAXP + 1];
   gll_points(ctx->p, x, w);
   for (int j = 0; j <= MAXP; j++) {

@@ -27,6 +27,9 @@ struct GeneralArgs {
   const int* code;    // FP32 storage: cell * (p+1)^D + anchor node (-1: identity row)
   const int* off;     //   fixed-point offsets from the anchor node (2^-30 units), D per record
   int dir_bytes;      // bytes of one direction's array (buffer-load range)
+  const int* wall_idx;        // walls: wall entry per DoF or -1 (null: no walls)
+  const unsigned* wall_flip;  //   per entry: bit alpha set when alpha reads the opposite direction
+  const double* wall_bias;    //   per entry: Q values added after interpolation
   double gll[MAXP + 1];
   double inv_den[MAXP + 1];
   double dgl[MAXP + 1][MAXP + 1];  // x_a - x_k
@@ -98,6 +101,12 @@ __device__ __forceinline__ void load_row(__amdgpu_buffer_rsrc_t r, int off, doub
 
 // Weights, cut-off and arithmetic in FP64 for both storage precisions (FP32 weights
 // rounded the same way at every step bias the FP32-shifted result).
+//
+// Velocity-neq bounce-back (SemiLagrangian.cpp:358-385, VelocityNeqBounceBack.cpp:
+// 137-195): a DoF whose path met a wall reads the opposite direction's populations
+// at the reflected departure point and adds 2 w (e . u_wall) / cs2 per hit. Such
+// rows take the term-by-term path with per-lane source addresses. With FP32
+// storage the shift w rho0 passes through, since opposite directions share w.
 template <int D, int Q, int P, typename T>
 __global__ __launch_bounds__(256) void k_sl_general(GeneralArgs<T> a, CollideArgs<T> ca, int write_macro) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -107,16 +116,25 @@ __global__ __launch_bounds__(256) void k_sl_general(GeneralArgs<T> a, CollideArg
   T fs[Q];
   fs[0] = a.fin[i];
   const int n0 = a.nfe[0], n1 = a.nfe[1];
+  int widx = -1;
+  unsigned wmask = 0;
+  if (a.wall_idx) {
+    widx = a.wall_idx[i];
+    if (widx >= 0) wmask = a.wall_flip[widx];
+  }
 #pragma unroll
   for (int al = 1; al < Q; al++) {
     const int64_t r = (int64_t)(al - 1) * a.n + i;
-    const T* src = a.fin + (int64_t)al * a.stride;
+    const T* own = a.fin + (int64_t)al * a.stride;
+    const bool flip = (wmask >> al) & 1u;
+    const T* src = flip ? a.fin + (int64_t)opposite<Q>(al) * a.stride : own;
+    const double bias = widx >= 0 ? a.wall_bias[(int64_t)widx * Q + al] : 0.0;
     int c;
     double lx[NP], ly[NP], lz[NP];
     if constexpr (sizeof(T) == 8) {
       c = a.cell[r];
       if (c < 0) {  // tracker life-time limit: identity row
-        fs[al] = src[i];
+        fs[al] = (T)((double)own[i] + bias);
         continue;
       }
       basis_xi<P>(a.gll, a.inv_den, a.xi[r * D + 0], lx);
@@ -125,7 +143,7 @@ __global__ __launch_bounds__(256) void k_sl_general(GeneralArgs<T> a, CollideArg
     } else {
       const int code = a.code[r];
       if (code < 0) {
-        fs[al] = src[i];
+        fs[al] = (T)((double)own[i] + bias);
         continue;
       }
       c = code / NL;
@@ -155,8 +173,8 @@ __global__ __launch_bounds__(256) void k_sl_general(GeneralArgs<T> a, CollideArg
       }
       mn = mx_ * my_ * mz_;
     }
-    if (mn >= 1e-10) {
-      const uint64_t pb = (uint64_t)src;
+    if (mn >= 1e-10 && !flip) {
+      const uint64_t pb = (uint64_t)own;
       const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pb >> 32));
       const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pb);
       const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
@@ -196,7 +214,7 @@ __global__ __launch_bounds__(256) void k_sl_general(GeneralArgs<T> a, CollideArg
             if (!(fabs(w) < 1e-10)) acc += w * (double)src[base + mx + (int64_t)n0 * (my + (int64_t)n1 * mz)];
           }
     }
-    fs[al] = (T)acc;
+    fs[al] = (T)(acc + bias);
   }
   bgk_collide_store<Q, T>(fs, ca, i, write_macro != 0, a.fout, a.stride);
 }
@@ -252,6 +270,9 @@ static void fill_general(nat_ctx* ctx, GeneralArgs<T>& a) {
   a.code = ctx->dep_code32;
   a.off = ctx->dep_off32;
   a.dir_bytes = (int)(ctx->stride * (int64_t)sizeof(T));
+  a.wall_idx = ctx->has_walls ? ctx->wall_idx : nullptr;
+  a.wall_flip = ctx->wall_flip;
+  a.wall_bias = ctx->wall_bias;
   double x[MAXP + 1], w[MAXP + 1];
   gll_points(ctx->p, x, w);
   for (int j = 0; j <= MAXP; j++) {

@@ -12,7 +12,9 @@
 #include "nat_internal.hpp"
 
 // kernel launchers (k_*.hip)
-int nat_launch_assemble_general(nat_ctx* ctx, const double* d_disp, int* d_cell, double* d_xi);
+int nat_launch_assemble_general(nat_ctx* ctx, const double* d_disp, const double* d_dirw, int* d_cell, double* d_xi,
+                                unsigned* d_flipmask, double* d_bias);
+int nat_launch_wall_compact(nat_ctx* ctx, const int* d_list, const unsigned* d_flipmask, const double* d_bias);
 int nat_launch_assemble_axis(nat_ctx* ctx, const int* d_N, const double* d_vaxis, int maxc, const double* d_disp,
                              const int* d_nshift, double* d_w, int* d_off);
 int nat_launch_records_f32(nat_ctx* ctx, int64_t nrec);
@@ -231,7 +233,7 @@ int check_status(nat_ctx* ctx) {
   NAT_HIP(hipStreamSynchronize(ctx->stream));
   if (st) NAT_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
   if (st & 1) return fail(ctx, NAT_ERR_DENSITY, "Densities too small (< 1e-10) for collisions. Decrease time step size.");
-  if (st & 2) return fail(ctx, NAT_ERR_UNSUPPORTED, "departure path reached a non-periodic boundary (walls are not supported)");
+  if (st & 2) return fail(ctx, NAT_ERR_UNSUPPORTED, "departure path reached a wall but no wall table was built");
   if (st & 4) return fail(ctx, NAT_ERR_ARG, "departure point could not be located in its cell");
   if (st & 16) return fail(ctx, NAT_ERR_ARG, "structured departure walk left the axis");
   return 0;
@@ -433,28 +435,85 @@ bool structured_instantiated(int D, int Q, int p) {
   return Q == 9 && pq;
 }
 
+// Device scratch freed when the set-up function returns, on every path.
+struct DevScratch {
+  std::vector<void*> bufs;
+  ~DevScratch() {
+    for (void* b : bufs)
+      if (b) hipFree(b);
+  }
+  template <typename T>
+  int get(nat_ctx* ctx, T** p, size_t count) {
+    int rc = alloc(ctx, p, count);
+    if (!rc) bufs.push_back(*p);
+    return rc;
+  }
+};
+
 int assemble_general(nat_ctx* ctx) {
   const int Qm1 = ctx->Q - 1;
-  std::vector<double> disp(3 * Qm1, 0.0);
-  for (int a = 1; a < ctx->Q; a++)
-    for (int d = 0; d < 3; d++) disp[3 * (a - 1) + d] = ctx->st.dir[a][d] * (-ctx->dt);
-  double* ddisp;
-  TRY(alloc(ctx, &ddisp, disp.size()));
+  std::vector<double> disp(3 * Qm1, 0.0), dirw(4 * Qm1, 0.0);
+  for (int a = 1; a < ctx->Q; a++) {
+    for (int d = 0; d < 3; d++) {
+      disp[3 * (a - 1) + d] = ctx->st.dir[a][d] * (-ctx->dt);
+      dirw[4 * (a - 1) + d] = ctx->st.dir[a][d];
+    }
+    dirw[4 * (a - 1) + 3] = ctx->st.w[a];
+  }
+  DevScratch tmp;
+  double *ddisp, *ddirw, *dbias = nullptr;
+  unsigned* dmask = nullptr;
+  TRY(tmp.get(ctx, &ddisp, disp.size()));
+  TRY(tmp.get(ctx, &ddirw, dirw.size()));
   NAT_HIP(hipMemcpyAsync(ddisp, disp.data(), disp.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  NAT_HIP(hipMemcpyAsync(ddirw, dirw.data(), dirw.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   const int64_t nrec = ctx->n_dofs * Qm1;
   if (!ctx->dep_cell) TRY(alloc(ctx, &ctx->dep_cell, nrec));
   if (!ctx->dep_xi64) TRY(alloc(ctx, &ctx->dep_xi64, nrec * ctx->D));
-  TRY(nat_launch_assemble_general(ctx, ddisp, ctx->dep_cell, ctx->dep_xi64));
+  if (ctx->has_walls) {
+    TRY(tmp.get(ctx, &dmask, ctx->n_dofs));
+    TRY(tmp.get(ctx, &dbias, nrec));
+    NAT_HIP(hipMemsetAsync(dmask, 0, sizeof(unsigned) * ctx->n_dofs, ctx->stream));
+  }
+  TRY(nat_launch_assemble_general(ctx, ddisp, ddirw, ctx->dep_cell, ctx->dep_xi64, dmask, dbias));
   if (ctx->fp32) {
     if (!ctx->dep_off32) TRY(alloc(ctx, &ctx->dep_off32, nrec * ctx->D));
     if (!ctx->dep_code32) TRY(alloc(ctx, &ctx->dep_code32, nrec));
-    const int64_t nl = ctx->D == 3 ? (int64_t)(ctx->p + 1) * (ctx->p + 1) * (ctx->p + 1) : (int64_t)(ctx->p + 1) * (ctx->p + 1);
+    const int64_t nl =
+        ctx->D == 3 ? (int64_t)(ctx->p + 1) * (ctx->p + 1) * (ctx->p + 1) : (int64_t)(ctx->p + 1) * (ctx->p + 1);
     if ((int64_t)ctx->desc.ncells[0] * ctx->desc.ncells[1] * ctx->desc.ncells[2] * nl >= (int64_t)1 << 31)
       return fail(ctx, NAT_ERR_UNSUPPORTED, "mesh too large for 32-bit FP32 departure records");
     TRY(nat_launch_records_f32(ctx, nrec));
   }
+  if (ctx->has_walls) {
+    // wall table: every node with a hit in some direction gets an entry
+    std::vector<unsigned> mask(ctx->n_dofs);
+    NAT_HIP(hipMemcpyAsync(mask.data(), dmask, sizeof(unsigned) * ctx->n_dofs, hipMemcpyDeviceToHost, ctx->stream));
+    NAT_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<int> idx(ctx->n_dofs, -1), list;
+    for (int64_t i = 0; i < ctx->n_dofs; i++)
+      if (mask[i]) {
+        idx[i] = (int)list.size();
+        list.push_back((int)i);
+      }
+    for (void* b : {(void*)ctx->wall_idx, (void*)ctx->wall_flip, (void*)ctx->wall_bias})
+      if (b) hipFree(b);
+    ctx->wall_idx = nullptr;
+    ctx->wall_flip = nullptr;
+    ctx->wall_bias = nullptr;
+    ctx->n_wall = (int64_t)list.size();
+    TRY(alloc(ctx, &ctx->wall_idx, ctx->n_dofs));
+    NAT_HIP(hipMemcpyAsync(ctx->wall_idx, idx.data(), sizeof(int) * ctx->n_dofs, hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->n_wall) {
+      int* dlist;
+      TRY(tmp.get(ctx, &dlist, list.size()));
+      TRY(alloc(ctx, &ctx->wall_flip, list.size()));
+      TRY(alloc(ctx, &ctx->wall_bias, list.size() * ctx->Q));
+      NAT_HIP(hipMemcpyAsync(dlist, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, ctx->stream));
+      TRY(nat_launch_wall_compact(ctx, dlist, dmask, dbias));
+    }
+  }
   NAT_HIP(hipStreamSynchronize(ctx->stream));
-  hipFree(ddisp);
   return check_status(ctx);
 }
 
@@ -480,7 +539,8 @@ void nat_destroy(nat_ctx* ctx) {
   void* bufs[] = {ctx->f[0],     ctx->f[1],        ctx->g[0],     ctx->g[1],       ctx->rho,       ctx->u,
                   ctx->T,        ctx->mask,        ctx->d_vert,   ctx->d_gll,      ctx->d_coords,  ctx->d_owner,
                   ctx->d_status, ctx->d_sums,      ctx->dep_cell, ctx->dep_xi64,   ctx->dep_off32, ctx->dep_code32, ctx->ax_w,
-                  ctx->ax_off,   ctx->sedg_df,     ctx->sedg_geo, ctx->comp_tab};
+                  ctx->ax_off,   ctx->sedg_df,     ctx->sedg_geo, ctx->comp_tab,    ctx->wall_idx,
+                  ctx->wall_flip, ctx->wall_bias};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -526,7 +586,14 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
       d.periodic[a] = 1;
     }
     if (d.ncells[a] < 1) return fail(ctx, NAT_ERR_ARG, "ncells must be >= 1");
-    if (!d.periodic[a]) return fail(ctx, NAT_ERR_UNSUPPORTED, "non-periodic (wall) boundaries are not supported");
+    if (!d.periodic[a]) ctx->has_walls = true;
+  }
+  if (ctx->has_walls) {
+    // velocity-neq bounce-back walls are built for SL streaming (SemiLagrangian.cpp:358-385)
+    if (d.scheme != NAT_SL) return fail(ctx, NAT_ERR_UNSUPPORTED, "walls are supported with SL streaming only");
+    if (d.backend == NAT_BACKEND_STRUCTURED)
+      return fail(ctx, NAT_ERR_UNSUPPORTED, "the structured backend needs periodic boundaries on every axis");
+    d.backend = NAT_BACKEND_GENERAL;
   }
   const int nz = d.dim == 3 ? d.ncells[2] + 1 : 1;
   const size_t nv = (size_t)(d.ncells[0] + 1) * (d.ncells[1] + 1) * nz;

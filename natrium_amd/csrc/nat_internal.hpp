@@ -79,6 +79,7 @@ struct MeshDev {
   int periodic[3];
   int cartesian;
   const double* vert;  // (N0+1)(N1+1)(N2+1) x 3 (z = 0 in 2D)
+  double wall_u[6][3] = {};  // velocity-neq bounce-back wall velocity per box face (non-periodic axes)
 };
 
 struct GllTab {
@@ -140,6 +141,15 @@ struct nat_ctx {
   double* dep_xi64 = nullptr;  // D per record
   int* dep_off32 = nullptr;    // FP32 storage: fixed-point offset from the anchor node (2^-30 units), D per record
   int* dep_code32 = nullptr;   // FP32: cell * (p+1)^D + anchor
+
+  // velocity-neq bounce-back walls (general SL backend): per node the index of its wall
+  // entry or -1; per wall entry a mask of the directions read from the opposite
+  // direction (bit alpha) and the value added per direction after interpolation
+  int* wall_idx = nullptr;
+  uint32_t* wall_flip = nullptr;
+  double* wall_bias = nullptr;  // n_wall x Q
+  int64_t n_wall = 0;
+  bool has_walls = false;
 
   // SL structured backend: 1-D tables
   void* ax_w = nullptr;   // [3][MAXSHIFT][MAXP][MAXP+1] weights (T)

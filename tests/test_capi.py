@@ -29,3 +29,41 @@ def test_library_exports_all_declared_symbols():
 def test_binding_loads_without_gpu():
     L = nat.lib()
     assert L.nat_last_error(None) == b"null context"
+
+
+def test_descriptor_layout_matches_header(tmp_path):
+    """The ctypes mirror of nat_desc has the C struct's size and field offsets."""
+    import shutil
+    import subprocess
+
+    import pytest
+
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    fields = [f[0] for f in nat._Desc._fields_]
+    prog = tmp_path / "desc.c"
+    body = "".join(f'  printf("%zu\\n", offsetof(nat_desc, {f}));\n' for f in fields)
+    prog.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "natrium_amd.h"\nint main(void) {\n'
+                    '  printf("%zu\\n", sizeof(nat_desc));\n' + body + "  return 0;\n}\n")
+    exe = tmp_path / "desc"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(nat._Desc)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(nat._Desc, f).offset == off, f
+
+
+def test_walls_rejected_before_device_use():
+    """Descriptor checks run before any device call: walls need SL streaming and the
+    general backend."""
+    import numpy as np
+    import pytest
+
+    from natrium_amd.problems import box_vertices
+
+    v = box_vertices(2, [3, 3], [0, 0], [1, 1])
+    with pytest.raises(nat.NatError, match="SL streaming"):
+        nat.Solver("D2Q9", 2, [3, 3], v, periodic=[1, 0], scheme="SEDG")
+    with pytest.raises(nat.NatError, match="periodic boundaries on every axis"):
+        nat.Solver("D2Q9", 2, [3, 3], np.asarray(v), periodic=[1, 0], backend="structured")

@@ -206,3 +206,27 @@ def test_channel_rows_closed_form(stencil, p, N, L, disp_layers):
             assert abs((0.0 if b is None else b[i]) - bias) < 1e-14 * max(1.0, abs(bias)), (a, i)
             n_flip += flip
     assert n_flip > 0
+
+
+def test_closed_box_corner_paths_hit_life_time_limit():
+    """Closed box, D2Q9: at each corner the two diagonal directions perpendicular to the corner's
+    inward bisector reverse at one wall, then at the other, and so on, until the tracker's
+    life-time limit (50 face steps, SemiLagrangian.cpp:254-279) writes an identity row. Every
+    other row of the box finds its departure point: 4 corners x 2 directions."""
+    v = box_vertices(2, [3, 3], [0, 0], [1, 1])
+    o = O.Oracle("D2Q9", 2, [3, 3], v, cartesian=True, periodic=[0, 0], scaling=1.0, viscosity=0.1)
+    o.assemble()
+    assert o.identity_rows() == 8
+    x = o.coords()
+    e, _, _, _ = O.stencil("D2Q9", 1.0)
+    got = set()
+    for a in range(1, 9):
+        for i in range(o.n):
+            if o.locate(i, a)[0] == -1:
+                got.add((tuple(float(c) for c in np.round(x[i], 12)), tuple(float(c) for c in e[a])))
+    want = set()
+    for corner in [(0.0, 0.0), (1.0, 0.0), (0.0, 1.0), (1.0, 1.0)]:
+        bx, by = (1.0 if corner[0] == 0 else -1.0), (1.0 if corner[1] == 0 else -1.0)
+        want.add((corner, (bx, -by)))
+        want.add((corner, (-bx, by)))
+    assert got == want
