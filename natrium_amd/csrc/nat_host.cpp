@@ -1004,12 +1004,17 @@ int64_t nat_internal_footprint(const nat_desc* d) {
     n_store = ncell * (D == 3 ? np * np * np : np * np);
     recs = Q * ((n_store + 127) / 128 * 128) * es;  // RK5 df
   } else {
-    const bool structured = !d->compressible && d->cartesian && d->backend != NAT_BACKEND_GENERAL &&
+    bool walls = false;
+    for (int a = 0; a < D; a++) walls = walls || !d->periodic[a];
+    const bool structured = !d->compressible && !walls && d->cartesian && d->backend != NAT_BACKEND_GENERAL &&
                             structured_instantiated(D, Q, p);
     n_store = structured ? nu : nfe;
     if (!structured) {
       recs = (int64_t)(Q - 1) * nfe * (4 + 8 * D);
       if (es == 4 && !d->compressible) recs += (int64_t)(Q - 1) * nfe * (4 + 4 * D);
+      // walls: node index per DoF, the wall entries (at most every node: mask + Q values)
+      // and, during assembly, a mask per DoF and one value per record
+      if (walls) recs += 4 * nfe + nfe * (4 + 8 * Q) + 4 * nfe + (int64_t)(Q - 1) * nfe * 8;
     }
   }
   const int64_t stride = (n_store + 127) / 128 * 128;

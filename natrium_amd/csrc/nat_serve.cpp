@@ -107,7 +107,8 @@ void build_problem(Req& q) {
     d.viscosity = U * 2 * M_PI / r.reynolds;  // TaylorGreenVortex2D: L = 2 pi
   } else if (r.problem == NAT_PROBLEM_TGV3D) {
     d.viscosity = U * 1.0 / r.reynolds;  // TaylorGreenVortex3D: characteristic length 1
-  } else {  // shock tube: unit-length strip, 4:1 cells
+  } else {  // shock tube: unit-length strip, 4:1 cells, walls at rest at both ends
+            // (SodShockTube.cpp:109-128: velocity-neq bounce-back on x, periodic across)
     L[0] = 1.0;
     L[1] = 0.25;
     L[2] = 0.25;
@@ -116,7 +117,7 @@ void build_problem(Req& q) {
   }
   for (int a = 0; a < 3; a++) {
     d.ncells[a] = a < d.dim ? n[a] : 1;
-    d.periodic[a] = 1;
+    d.periodic[a] = (r.problem == NAT_PROBLEM_SHOCK_TUBE && a == 0) ? 0 : 1;
   }
   const int nz = d.dim == 3 ? d.ncells[2] + 1 : 1;
   q.verts.clear();
