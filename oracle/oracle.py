@@ -69,6 +69,8 @@ def lib():
         L.orc_rk5_scalar.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.orc_set_compressible_options.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
         L.orc_set_walls.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_set_stabilizer.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_stabilizer_matrix.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
         L.orc_op_hits.argtypes = [ctypes.c_void_p, _lp, _lp]
         L.orc_op_src.restype = ctypes.c_int
         L.orc_op_src.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int8)]
@@ -118,6 +120,13 @@ def quartic_feq(name, scaling, rho, u, T):
     return out
 
 
+def stabilizer_matrix(name, scaling=1.0):
+    sid, D, Q = STENCILS[name]
+    P = np.zeros((Q, Q))
+    lib().orc_stabilizer_matrix(sid, scaling, _p(P))
+    return P
+
+
 def rk5_scalar(lam, f0, dt, nsteps):
     return lib().orc_rk5_scalar(lam, f0, dt, nsteps)
 
@@ -135,7 +144,7 @@ class Oracle:
 
     def __init__(self, stencil, p, ncells, vertices, cartesian=True, periodic=None, scaling=1.0, viscosity=1.0,
                  cfl=0.4, scheme="SL", compressible=False, gamma=1.4, prandtl=0.0, sutherland=False,
-                 wall_velocity=None):
+                 wall_velocity=None, stabilizer=False):
         """periodic: flag per axis (default all periodic). Faces of non-periodic axes are
         velocity-neq bounce-back walls; wall_velocity maps a box face 2*axis+side to its
         velocity (default at rest)."""
@@ -157,6 +166,7 @@ class Oracle:
         for f, v in (wall_velocity or {}).items():
             wu[int(f), :len(v)] = v
         lib().orc_set_walls(self.h, _p(wu))
+        lib().orc_set_stabilizer(self.h, int(stabilizer))
 
     def __del__(self):
         if getattr(self, "h", None):

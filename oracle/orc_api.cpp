@@ -35,6 +35,8 @@ struct orc_solver {
   SLOperator op;
   int64_t n = 0;
   std::vector<double> f, g, rho, u, T, mask, work, ftmp;
+  bool stabilizer = false;  // PseudoEntropicStabilizer data processor
+  std::vector<double> stab_P;
   std::string err;
   std::vector<double*> fptr(std::vector<double>& v, int Q) {
     std::vector<double*> r(Q);
@@ -247,13 +249,33 @@ int orc_stream(orc_solver* s) {
   return 0;
 }
 
-// n full steps (stream + collide); the caller runs the initial collide() once.
+// Append the pseudo-entropic stabilizer as a data processor (CFDSolver::appendDataProcessor):
+// applied after the collision of every loop iteration, for D2Q9 and D3Q19 only
+// (PseudoEntropicStabilizer.cpp:271-289 does nothing for other stencils).
+void orc_set_stabilizer(orc_solver* s, int on) {
+  s->stabilizer = on != 0;
+  if (s->stabilizer) stabilizer_matrix(s->st, s->stab_P);
+}
+
+// n full steps (stream + collide + data processors, CFDSolver.cpp:877-892); the caller runs
+// the initial collide() once.
 int orc_steps(orc_solver* s, int nsteps) {
   for (int i = 0; i < nsteps; i++) {
     orc_stream(s);
     if (orc_collide(s)) return 1;
+    if (s->stabilizer && (s->st.Q == 9 || s->st.Q == 19)) {
+      auto fp = s->fptr(s->f, s->st.Q);
+      apply_stabilizer(s->st, s->stab_P, s->n, fp.data());
+    }
   }
   return 0;
+}
+
+void orc_stabilizer_matrix(int id, double scaling, double* P) {
+  Stencil st = make_stencil(id, scaling);
+  std::vector<double> m;
+  stabilizer_matrix(st, m);
+  std::memcpy(P, m.data(), sizeof(double) * m.size());
 }
 
 void orc_get_f(orc_solver* s, double* f) { std::memcpy(f, s->f.data(), sizeof(double) * s->f.size()); }

@@ -377,4 +377,50 @@ void quartic_feq(const Stencil& st, double rho, const double u[3], double T, dou
   delete h;
 }
 
+void stabilizer_matrix(const Stencil& st, std::vector<double>& P) {
+  const int Q = st.Q, D = st.D;
+  const double s = st.scaling, cs2 = 1.0 / 3.0;
+  // Hermite polynomials of order <= 2 at the unscaled velocities
+  std::vector<std::vector<double>> H;
+  std::vector<double> h(Q);
+  for (int i = 0; i < Q; i++) h[i] = 1.0;
+  H.push_back(h);
+  for (int a = 0; a < D; a++) {
+    for (int i = 0; i < Q; i++) h[i] = st.dir[i][a] / s;
+    H.push_back(h);
+  }
+  for (int a = 0; a < D; a++)
+    for (int b = a; b < D; b++) {
+      for (int i = 0; i < Q; i++) h[i] = (st.dir[i][a] / s) * (st.dir[i][b] / s) - (a == b ? cs2 : 0.0);
+      H.push_back(h);
+    }
+  const int K = (int)H.size();
+  // the basis is w-orthogonal on D2Q9 and D3Q19, so G is diagonal
+  std::vector<double> g(K, 0.0);
+  for (int k = 0; k < K; k++)
+    for (int i = 0; i < Q; i++) g[k] += st.w[i] * H[k][i] * H[k][i];
+  P.assign((size_t)Q * Q, 0.0);
+  for (int i = 0; i < Q; i++)
+    for (int j = 0; j < Q; j++) {
+      double v = 0;
+      for (int k = 0; k < K; k++) v += H[k][i] * H[k][j] / g[k];
+      P[(size_t)i * Q + j] = st.w[i] * v;
+    }
+}
+
+void apply_stabilizer(const Stencil& st, const std::vector<double>& P, int64_t n, double* const* f) {
+  const int Q = st.Q;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; i++) {
+    double in[32], out[32];
+    for (int q = 0; q < Q; q++) in[q] = f[q][i];
+    for (int r = 0; r < Q; r++) {
+      double v = 0;
+      for (int q = 0; q < Q; q++) v += P[(size_t)r * Q + q] * in[q];
+      out[r] = v;
+    }
+    for (int q = 0; q < Q; q++) f[q][i] = out[q];
+  }
+}
+
 }  // namespace orc

@@ -10,6 +10,7 @@
 //   collision_advanced/AuxiliaryCollisionFunctions.h:46-68,106-202,232-307,421-431,475-481,520-566
 //   collision/BGKStandard.cpp:23-41                 legacy equilibrium used for initial f
 //   solver/CompressibleCFDSolver.h:790-848,850-1004 compressible initial f, g
+//   dataprocessors/PseudoEntropicStabilizer.cpp:271-289 stabilizer after each loop collision
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -42,5 +43,14 @@ void init_compressible(const Stencil& st, const CollisionParams& cp, int64_t n, 
 
 // quartic equilibrium (polynomial form) for one node, e unscaled, cs2 unscaled
 void quartic_feq(const Stencil& st, double rho, const double u[3], double T, double* feq);
+
+// Pseudo-entropic stabilizer of D2Q9 / D3Q19 (PseudoEntropicStabilizer::apply): a fixed
+// linear map f <- P f per node, P the projection onto the populations spanned by
+// w_i H_k(e_i) for the Hermite polynomials of order <= 2 (1, e_a, e_a e_b - cs2 delta_ab),
+// orthogonal in the w-weighted inner product. It keeps density, momentum and the second
+// moments and removes the higher (ghost) moments. Built here from the unscaled stencil:
+// P = diag(w) H G^-1 H^T with G = H^T diag(w) H. Q x Q row-major.
+void stabilizer_matrix(const Stencil& st, std::vector<double>& P);
+void apply_stabilizer(const Stencil& st, const std::vector<double>& P, int64_t n, double* const* f);
 
 }  // namespace orc
