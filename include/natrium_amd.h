@@ -97,6 +97,13 @@ int nat_node_coords(nat_ctx* ctx, double* xyz /* n_dofs x 3 */);
  * weights are located on the GPU in FP64. dt <= 0 keeps the CFL time step. */
 int nat_set_dt(nat_ctx* ctx, double dt);
 
+/* CFDSolver::appendDataProcessor(PseudoEntropicStabilizer) (solver/CFDSolver.h:69-373,
+ * dataprocessors/PseudoEntropicStabilizer.cpp:271-311): on = 1 applies the stabilizer after the
+ * collision of every nat_step iteration (not after nat_collide, as run() applies data
+ * processors after its loop collide only). Like the reference it acts on D2Q9 and D3Q19 and
+ * leaves other stencils unchanged. */
+int nat_set_stabilizer(nat_ctx* ctx, int on);
+
 /* CFDSolver::initializeDistributions (CFDSolver.cpp:1104-1129; compressible:
  * CompressibleCFDSolver.h:850-946): f = equilibrium(rho, u[, T]); T may be NULL. */
 int nat_init_equilibrium(nat_ctx* ctx, const double* rho, const double* u, const double* T);

@@ -45,7 +45,8 @@ class _Desc(ctypes.Structure):
 
 EXPORTS = [
     "nat_create", "nat_destroy", "nat_last_error", "nat_n_dofs", "nat_n_stored_nodes", "nat_Q", "nat_dt",
-    "nat_tau", "nat_time", "nat_backend", "nat_node_coords", "nat_set_dt", "nat_init_equilibrium",
+    "nat_tau", "nat_time", "nat_backend", "nat_node_coords", "nat_set_dt", "nat_set_stabilizer",
+    "nat_init_equilibrium",
     "nat_upload_f", "nat_download_f", "nat_upload_g", "nat_download_g", "nat_collide", "nat_stream",
     "nat_step", "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sync",
 ]
@@ -82,6 +83,8 @@ def lib():
     L.nat_node_coords.argtypes = [vp, _dp]
     L.nat_set_dt.restype = ctypes.c_int
     L.nat_set_dt.argtypes = [vp, ctypes.c_double]
+    L.nat_set_stabilizer.restype = ctypes.c_int
+    L.nat_set_stabilizer.argtypes = [vp, ctypes.c_int]
     L.nat_init_equilibrium.restype = ctypes.c_int
     L.nat_init_equilibrium.argtypes = [vp, _dp, _dp, _dp]
     for nm in ("nat_upload_f", "nat_download_f", "nat_upload_g", "nat_download_g"):
@@ -115,7 +118,7 @@ class Solver:
     def __init__(self, stencil, order, ncells, vertices, cartesian=True, periodic=None, scaling=1.0,
                  viscosity=1.0, cfl=0.4, scheme="SL", precision="fp64", backend="auto", rho0=1.0,
                  compressible=False, gamma=1.4, equilibrium=None, prandtl=0.0, sutherland=False,
-                 wall_velocity=None):
+                 wall_velocity=None, stabilizer=False):
         """periodic: flag per axis (default: all periodic). Both faces of a non-periodic axis
         are velocity-neq bounce-back walls (SL only); wall_velocity maps a box face
         2*axis+side (side 0 = lower) to its constant velocity (default: at rest)."""
@@ -161,6 +164,8 @@ class Solver:
         self.n = L.nat_n_dofs(h)
         self.Q = L.nat_Q(h)
         self.compressible = compressible
+        if stabilizer:
+            self.set_stabilizer(True)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -209,6 +214,10 @@ class Solver:
 
     def set_dt(self, dt):
         self._check(lib().nat_set_dt(self._h, dt))
+
+    def set_stabilizer(self, on=True):
+        """Pseudo-entropic stabilizer after every step's collision (D2Q9 / D3Q19)."""
+        self._check(lib().nat_set_stabilizer(self._h, int(on)))
 
     def init_equilibrium(self, rho, u, T=None):
         rho = np.ascontiguousarray(rho, dtype=np.float64)

@@ -278,6 +278,7 @@ static int sedg_step_typed(nat_ctx* ctx, int collide_after, int write_macro) {
   ca.u = (T*)ctx->u;
   ca.mstride = ctx->n_store;
   ca.status = ctx->d_status;
+  ca.stab = ctx->stab_on();
   for (int st = 0; st < 5; st++) {
     s.fin = (const T*)ctx->f[ctx->cur];
     s.fout = (T*)ctx->f[1 - ctx->cur];

@@ -308,6 +308,7 @@ int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
     ca.u = (float*)ctx->u;
     ca.mstride = ctx->n_store;
     ca.status = ctx->d_status;
+    ca.stab = ctx->stab_on();
     rc = dispatch_general<float>(ctx, a, ca, write_macro);
   } else {
     GeneralArgs<double> a;
@@ -321,6 +322,7 @@ int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
     ca.u = (double*)ctx->u;
     ca.mstride = ctx->n_store;
     ca.status = ctx->d_status;
+    ca.stab = ctx->stab_on();
     rc = dispatch_general<double>(ctx, a, ca, write_macro);
   }
   if (rc == 0) ctx->cur = dst;

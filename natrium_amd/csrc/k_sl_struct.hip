@@ -538,6 +538,7 @@ static int launch_typed(nat_ctx* ctx, int write_macro) {
   ca.u = (T*)ctx->u;
   ca.mstride = ctx->n_store;
   ca.status = ctx->d_status;
+  ca.stab = ctx->stab_on();
   int rc = dispatch_struct<T>(ctx, a, ca, write_macro);
   if (rc == 0) ctx->cur = dst;
   return rc;

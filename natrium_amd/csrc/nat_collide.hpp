@@ -22,6 +22,74 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
+// Pseudo-entropic stabilizer (dataprocessors/PseudoEntropicStabilizer.cpp:271-311), applied
+// after the collision of each loop step (solver/CFDSolver.cpp:877-892): the post-collision
+// populations are projected onto w times the Hermite polynomials of order <= 2, the
+// w-orthogonal projection P with P f_eq = f_eq. Since P keeps density and momentum,
+// P (f - omega (f - f_eq)) = f_eq + (1 - omega) P f_neq with
+// (P f_neq)_i = w_i (e_i e_i - cs2 I) : Pi_neq / (2 cs2^2), Pi_neq = sum_j e_j e_j f_neq_j
+// = Pi(f) - rho (u u + cs2 I). In the shifted variables Pi(f~) = Pi(f) - rho0 cs2 I, so
+// Pi_neq = Pi(f~) - rho u u - drho cs2 I. Unscaled units: cs2 = 1/3.
+template <int Q>
+struct StabPi {
+  double xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
+};
+
+// second moments of the populations (e_i integer)
+template <int Q>
+__device__ __forceinline__ StabPi<Q> stab_moments(const double (&f)[Q]) {
+  using L = Lattice<Q>;
+  StabPi<Q> m;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const int ex = L::e[i][0], ey = L::e[i][1], ez = L::e[i][2];
+    if (ex) m.xx += f[i];
+    if (ey) m.yy += f[i];
+    if (ez) m.zz += f[i];
+    if (ex * ey > 0) m.xy += f[i];
+    if (ex * ey < 0) m.xy -= f[i];
+    if (ex * ez > 0) m.xz += f[i];
+    if (ex * ez < 0) m.xz -= f[i];
+    if (ey * ez > 0) m.yz += f[i];
+    if (ey * ez < 0) m.yz -= f[i];
+  }
+  return m;
+}
+
+// Pi_neq from the shifted moments, scaled by (1 - omega) 9/2 so that the correction of
+// direction i is w_i (sum_ab e_a e_b P_ab - tr P / 3)
+template <int Q>
+__device__ __forceinline__ StabPi<Q> stab_neq(StabPi<Q> m, double drho, double rho, const double (&u)[3],
+                                              double om) {
+  const double k = (1.0 - om) * 4.5;
+  const double c = drho / 3.0;
+  m.xx = k * (m.xx - rho * u[0] * u[0] - c);
+  m.yy = k * (m.yy - rho * u[1] * u[1] - c);
+  m.zz = k * (m.zz - rho * u[2] * u[2] - (Lattice<Q>::D == 3 ? c : 0.0));
+  m.xy = k * (m.xy - rho * u[0] * u[1]);
+  m.xz = k * (m.xz - rho * u[0] * u[2]);
+  m.yz = k * (m.yz - rho * u[1] * u[2]);
+  return m;
+}
+
+template <int Q>
+__device__ __forceinline__ double stab_term(const StabPi<Q>& m, int i) {
+  using L = Lattice<Q>;
+  const int ex = L::e[i][0], ey = L::e[i][1], ez = L::e[i][2];
+  double v = 0;
+  if (ex) v += m.xx;
+  if (ey) v += m.yy;
+  if (ez) v += m.zz;
+  if (ex * ey > 0) v += 2 * m.xy;
+  if (ex * ey < 0) v -= 2 * m.xy;
+  if (ex * ez > 0) v += 2 * m.xz;
+  if (ex * ez < 0) v -= 2 * m.xz;
+  if (ey * ez > 0) v += 2 * m.yz;
+  if (ey * ez < 0) v -= 2 * m.yz;
+  const double tr = (Lattice<Q>::D == 3) ? (m.xx + m.yy + m.zz) : (m.xx + m.yy);
+  return L::w[i] * (v - tr / 3.0);
+}
+
 // cs2 = 1/3 in unscaled units for D2Q9 / D3Q19 / D3Q27
 template <int Q, typename T>
 __device__ __forceinline__ void bgk_collide(T (&f)[Q], const CollideArgs<T>& ca, int64_t node, bool write_macro) {
@@ -149,18 +217,38 @@ __device__ __forceinline__ void bgk_collide_store(const T (&f)[Q], const Collide
   for (int d = 0; d < D; d++) uu += u[d] * u[d];
   const double c_uu = 1.5 * uu;
   const double om = ca.omega64;
+  if ((Q == 9 || Q == 19) && ca.stab) {
+    double fd[Q];
 #pragma unroll
-  for (int i = 0; i < Q; i++) {
-    double ue = 0;
+    for (int i = 0; i < Q; i++) fd[i] = (double)f[i];
+    double u3[3] = {u[0], u[1], D == 3 ? u[2] : 0.0};
+    const StabPi<Q> pn = stab_neq<Q>(stab_moments<Q>(fd), drho, rho, u3, om);
 #pragma unroll
-    for (int d = 0; d < D; d++) {
-      if (L::e[i][d] > 0) ue += u[d];
-      if (L::e[i][d] < 0) ue -= u[d];
+    for (int i = 0; i < Q; i++) {
+      double ue = 0;
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        if (L::e[i][d] > 0) ue += u[d];
+        if (L::e[i][d] < 0) ue -= u[d];
+      }
+      ue *= 3.0;
+      const double feq = L::w[i] * (drho + rho * (ue * (1.0 + 0.5 * ue) - c_uu));
+      fout[(int64_t)i * stride + node] = (T)(feq + stab_term<Q>(pn, i));
     }
-    ue *= 3.0;
-    const double feq = L::w[i] * (drho + rho * (ue * (1.0 + 0.5 * ue) - c_uu));
-    const double fi = (double)f[i];
-    fout[(int64_t)i * stride + node] = (T)(fi - om * (fi - feq));
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      double ue = 0;
+#pragma unroll
+      for (int d = 0; d < D; d++) {
+        if (L::e[i][d] > 0) ue += u[d];
+        if (L::e[i][d] < 0) ue -= u[d];
+      }
+      ue *= 3.0;
+      const double feq = L::w[i] * (drho + rho * (ue * (1.0 + 0.5 * ue) - c_uu));
+      const double fi = (double)f[i];
+      fout[(int64_t)i * stride + node] = (T)(fi - om * (fi - feq));
+    }
   }
   if (write_macro && ca.rho) {
     ca.rho[node] = (T)rho;
@@ -223,24 +311,48 @@ __device__ __forceinline__ void bgk_collide_store_pairs(const T (&f)[Q], const C
     char* p = (char*)(fout + (int64_t)i * stride);
     *(T*)(p + nb) = (T)v;
   };
-  store(0, fd[0] - om * (fd[0] - L::w[0] * base));
-  static_for<1, Q>([&](auto I) {
-    constexpr int i = decltype(I)::value;
-    constexpr int o = opposite<Q>(i);
-    if constexpr (o > i) {
-      double ue = 0;
+  if ((Q == 9 || Q == 19) && ca.stab) {
+    // stabilized: f_eq + (1 - omega) P f_neq; the correction is even in e
+    double uu3[3] = {u[0], u[1], D == 3 ? u[2] : 0.0};
+    const StabPi<Q> pn = stab_neq<Q>(stab_moments<Q>(fd), drho, rho, uu3, om);
+    store(0, L::w[0] * base + stab_term<Q>(pn, 0));
+    static_for<1, Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      constexpr int o = opposite<Q>(i);
+      if constexpr (o > i) {
+        double ue = 0;
 #pragma unroll
-      for (int d = 0; d < D; d++) {
-        if (L::e[i][d] > 0) ue += u3[d];
-        if (L::e[i][d] < 0) ue -= u3[d];
+        for (int d = 0; d < D; d++) {
+          if (L::e[i][d] > 0) ue += u3[d];
+          if (L::e[i][d] < 0) ue -= u3[d];
+        }
+        const double E = base + hr * (ue * ue);
+        const double O = rho * ue;
+        const double wE = L::w[i] * E + stab_term<Q>(pn, i), wO = L::w[i] * O;
+        store(i, wE + wO);
+        store(o, wE - wO);
       }
-      const double E = base + hr * (ue * ue);
-      const double O = rho * ue;
-      const double wE = L::w[i] * E, wO = L::w[i] * O;
-      store(i, fd[i] - om * (fd[i] - (wE + wO)));
-      store(o, fd[o] - om * (fd[o] - (wE - wO)));
-    }
-  });
+    });
+  } else {
+    store(0, fd[0] - om * (fd[0] - L::w[0] * base));
+    static_for<1, Q>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      constexpr int o = opposite<Q>(i);
+      if constexpr (o > i) {
+        double ue = 0;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          if (L::e[i][d] > 0) ue += u3[d];
+          if (L::e[i][d] < 0) ue -= u3[d];
+        }
+        const double E = base + hr * (ue * ue);
+        const double O = rho * ue;
+        const double wE = L::w[i] * E, wO = L::w[i] * O;
+        store(i, fd[i] - om * (fd[i] - (wE + wO)));
+        store(o, fd[o] - om * (fd[o] - (wE - wO)));
+      }
+    });
+  }
   if (write_macro && ca.rho) {
     ca.rho[node] = (T)rho;
 #pragma unroll

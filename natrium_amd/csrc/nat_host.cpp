@@ -701,6 +701,11 @@ int nat_node_coords(nat_ctx* ctx, double* xyz) {
   return 0;
 }
 
+int nat_set_stabilizer(nat_ctx* ctx, int on) {
+  ctx->stabilizer = on != 0;
+  return 0;
+}
+
 int nat_set_dt(nat_ctx* ctx, double dt) {
   if (dt > 0) {
     ctx->dt = dt;

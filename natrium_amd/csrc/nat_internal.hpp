@@ -97,6 +97,7 @@ struct CollideArgs {
   T* u;        // D arrays, physical units
   int64_t mstride;
   int* status;  // bit 0: density < 1e-10 seen
+  int stab = 0;  // pseudo-entropic stabilizer after the collision (D2Q9 / D3Q19)
 };
 
 }  // namespace nat
@@ -150,6 +151,11 @@ struct nat_ctx {
   double* wall_bias = nullptr;  // n_wall x Q
   int64_t n_wall = 0;
   bool has_walls = false;
+
+  // pseudo-entropic stabilizer as a data processor after each loop step's collision
+  // (D2Q9 and D3Q19 only, as in the reference)
+  bool stabilizer = false;
+  int stab_on() const { return stabilizer && (Q == 9 || Q == 19) ? 1 : 0; }
 
   // SL structured backend: 1-D tables
   void* ax_w = nullptr;   // [3][MAXSHIFT][MAXP][MAXP+1] weights (T)
