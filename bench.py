@@ -97,7 +97,7 @@ def cpu_baseline(prob, seconds):
                       f"{o.n} FE_Q DoFs, {el:.1f} s wall, OpenMP {threads} threads; assembly untimed"}
 
 
-def extra_configs(nat, steps_sl=100, steps_err=500):
+def extra_configs(nat, steps_sl=100, steps_err=500, cpu=False):
     """Beside the headline: the other hot-path configurations, FP32-shifted and FP64 GLUPS on
     the device clock (HIP events), and FP32-shifted vs FP64 velocity errors.
 
@@ -107,6 +107,10 @@ def extra_configs(nat, steps_sl=100, steps_err=500):
         RK5 (884,736 DG nodes).
     C4: unit square, 128^2 quads, p=2, D2Q25H compressible f/g, Riemann quadrant states
         (seed 7), velocity-neq bounce-back walls at rest on all four sides, errors after 2000 steps.
+    Reassembly: the C3 mesh moved to a second perturbation (amplitude 0.12 h, seed 77) and the SL
+        operator rebuilt (FP64 departure walk through the Q1 map, 389,017 DoFs x 18 directions):
+        wall time of nat_set_vertices on the GPU and, with --cpu-assembly, of the oracle's
+        assembly of the same operator (about a minute and 7 GB of host memory).
     """
     import numpy as np
 
@@ -154,6 +158,25 @@ def extra_configs(nat, steps_sl=100, steps_err=500):
     mk_sl = lambda prec: nat.Solver("D3Q19", 3, [24] * 3, v3, cartesian=False, scaling=s, viscosity=0.01, cfl=0.4,
                                     precision=prec)
     out["c3_sl_general"] = run(mk_sl, init3, steps_sl, steps_err)
+    v3b = perturbed_vertices(3, [24] * 3, [0] * 3, [L] * 3, amplitude=0.12, seed=77)
+    sol = mk_sl("fp32")
+    init3(sol)
+    sol.sync()
+    t0 = time.perf_counter()
+    sol.set_vertices(v3b)
+    sol.sync()
+    out["c3_sl_general"]["reassembly_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    del sol
+    if cpu:
+        from oracle import oracle as O
+
+        O.set_threads(max(1, min(16, os.cpu_count() or 1)))
+        o = O.Oracle("D3Q19", 3, [24] * 3, v3b, cartesian=False, scaling=s, viscosity=0.01, cfl=0.4)
+        t0 = time.perf_counter()
+        o.assemble()
+        out["c3_sl_general"]["oracle_assembly_s"] = round(time.perf_counter() - t0, 2)
+        out["c3_sl_general"]["oracle_assembly_threads"] = O.get_threads()
+        del o
     mk_dg = lambda prec: nat.Solver("D3Q19", 3, [24] * 3, v3, cartesian=False, scaling=s, viscosity=0.01, cfl=0.4,
                                     precision=prec, scheme="SEDG")
     out["c3_sedg_rk5"] = run(mk_dg, init3, max(steps_sl // 5, 10), steps_err // 5)
@@ -256,6 +279,8 @@ def main():
     ap.add_argument("--error-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-assembly", action="store_true",
+                    help="also time the oracle's assembly of the moved C3 operator on the host")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-full", action="store_true", help="512 requests, full refinement and step ranges")
     ap.add_argument("--no-serve", action="store_true")
@@ -339,7 +364,7 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(prob, args.cpu_seconds)
 
-    extra = None if args.no_extra else extra_configs(nat)
+    extra = None if args.no_extra else extra_configs(nat, cpu=args.cpu_assembly)
 
     serving = None
     if not args.no_serve:

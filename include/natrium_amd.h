@@ -97,6 +97,14 @@ int nat_node_coords(nat_ctx* ctx, double* xyz /* n_dofs x 3 */);
  * weights are located on the GPU in FP64. dt <= 0 keeps the CFL time step. */
 int nat_set_dt(nat_ctx* ctx, double dt);
 
+/* Moved mesh, same lattice topology: the transformed triangulation followed by
+ * AdvectionOperator::reassemble (SemiLagrangian.cpp:76-90, the departure walk of
+ * SemiLagrangian.cpp:145-757 through MappingQ1 in FP64 on the GPU). The populations stay
+ * attached to their DoFs and dt is kept. A cartesian = 1 context needs axis-aligned cells;
+ * the structured backend needs a uniform Cartesian mesh. SEDG contexts read the new
+ * geometry in their next stage. */
+int nat_set_vertices(nat_ctx* ctx, const double* vertices);
+
 /* CFDSolver::appendDataProcessor(PseudoEntropicStabilizer) (solver/CFDSolver.h:69-373,
  * dataprocessors/PseudoEntropicStabilizer.cpp:271-311): on = 1 applies the stabilizer after the
  * collision of every nat_step iteration (not after nat_collide, as run() applies data

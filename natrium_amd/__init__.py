@@ -45,7 +45,7 @@ class _Desc(ctypes.Structure):
 
 EXPORTS = [
     "nat_create", "nat_destroy", "nat_last_error", "nat_n_dofs", "nat_n_stored_nodes", "nat_Q", "nat_dt",
-    "nat_tau", "nat_time", "nat_backend", "nat_node_coords", "nat_set_dt", "nat_set_stabilizer",
+    "nat_tau", "nat_time", "nat_backend", "nat_node_coords", "nat_set_dt", "nat_set_vertices", "nat_set_stabilizer",
     "nat_init_equilibrium",
     "nat_upload_f", "nat_download_f", "nat_upload_g", "nat_download_g", "nat_collide", "nat_stream",
     "nat_step", "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sync",
@@ -83,6 +83,8 @@ def lib():
     L.nat_node_coords.argtypes = [vp, _dp]
     L.nat_set_dt.restype = ctypes.c_int
     L.nat_set_dt.argtypes = [vp, ctypes.c_double]
+    L.nat_set_vertices.restype = ctypes.c_int
+    L.nat_set_vertices.argtypes = [vp, _dp]
     L.nat_set_stabilizer.restype = ctypes.c_int
     L.nat_set_stabilizer.argtypes = [vp, ctypes.c_int]
     L.nat_init_equilibrium.restype = ctypes.c_int
@@ -214,6 +216,14 @@ class Solver:
 
     def set_dt(self, dt):
         self._check(lib().nat_set_dt(self._h, dt))
+
+    def set_vertices(self, vertices):
+        """Move the mesh (same lattice) and reassemble the operator on the device."""
+        v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, self.D)
+        if v.shape != self._verts.shape:
+            raise ValueError("vertex array of a different lattice")
+        self._check(lib().nat_set_vertices(self._h, _ptr(v)))
+        self._verts = v
 
     def set_stabilizer(self, on=True):
         """Pseudo-entropic stabilizer after every step's collision (D2Q9 / D3Q19)."""

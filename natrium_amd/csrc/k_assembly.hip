@@ -23,66 +23,72 @@ __device__ __forceinline__ const double* vtx(const MeshDev& m, int i, int j, int
   return m.vert + 3 * ((size_t)i + (size_t)(m.N[0] + 1) * ((size_t)j + (size_t)(m.N[1] + 1) * k));
 }
 
+template <int D>
 __device__ __forceinline__ void cell_xyz(const MeshDev& m, int cell, int c[3]) {
   c[0] = cell % m.N[0];
   c[1] = (cell / m.N[0]) % m.N[1];
-  c[2] = m.D == 3 ? cell / (m.N[0] * m.N[1]) : 0;
+  c[2] = D == 3 ? cell / (m.N[0] * m.N[1]) : 0;
 }
 
+template <int D>
 __device__ __forceinline__ const double* cell_vertex(const MeshDev& m, const int c[3], int v) {
-  return vtx(m, c[0] + (v & 1), c[1] + ((v >> 1) & 1), m.D == 3 ? c[2] + ((v >> 2) & 1) : 0);
+  return vtx(m, c[0] + (v & 1), c[1] + ((v >> 1) & 1), D == 3 ? c[2] + ((v >> 2) & 1) : 0);
 }
 
+template <int D>
 __device__ void map_fwd(const MeshDev& m, const int c[3], const double xi[3], double x[3]) {
-  const int nv = m.D == 3 ? 8 : 4;
+  const int nv = D == 3 ? 8 : 4;
   if (m.cartesian) {
-    const double* a = cell_vertex(m, c, 0);
-    const double* b = cell_vertex(m, c, nv - 1);
-    for (int d = 0; d < 3; d++) x[d] = d < m.D ? a[d] + xi[d] * (b[d] - a[d]) : 0.0;
+    const double* a = cell_vertex<D>(m, c, 0);
+    const double* b = cell_vertex<D>(m, c, nv - 1);
+    for (int d = 0; d < 3; d++) x[d] = d < D ? a[d] + xi[d] * (b[d] - a[d]) : 0.0;
     return;
   }
   x[0] = x[1] = x[2] = 0;
   for (int v = 0; v < nv; v++) {
     double Nv = 1;
-    for (int a = 0; a < m.D; a++) Nv *= ((v >> a) & 1) ? xi[a] : (1 - xi[a]);
-    const double* X = cell_vertex(m, c, v);
-    for (int d = 0; d < m.D; d++) x[d] += Nv * X[d];
+    for (int a = 0; a < D; a++) Nv *= ((v >> a) & 1) ? xi[a] : (1 - xi[a]);
+    const double* X = cell_vertex<D>(m, c, v);
+    for (int d = 0; d < D; d++) x[d] += Nv * X[d];
   }
 }
 
+template <int D>
 __device__ void jac(const MeshDev& m, const int c[3], const double xi[3], double J[3][3]) {
   for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) J[a][b] = (a == b && a >= m.D) ? 1.0 : 0.0;
-  const int nv = m.D == 3 ? 8 : 4;
+    for (int b = 0; b < 3; b++) J[a][b] = (a == b && a >= D) ? 1.0 : 0.0;
+  const int nv = D == 3 ? 8 : 4;
   for (int v = 0; v < nv; v++) {
-    const double* X = cell_vertex(m, c, v);
-    for (int b = 0; b < m.D; b++) {
+    const double* X = cell_vertex<D>(m, c, v);
+    for (int b = 0; b < D; b++) {
       double dN = 1;
-      for (int a = 0; a < m.D; a++) {
+      for (int a = 0; a < D; a++) {
         const int bit = (v >> a) & 1;
         dN *= (a == b) ? (bit ? 1.0 : -1.0) : (bit ? xi[a] : 1 - xi[a]);
       }
-      for (int d = 0; d < m.D; d++) J[d][b] += dN * X[d];
+      for (int d = 0; d < D; d++) J[d][b] += dN * X[d];
     }
   }
 }
 
 // inverse map; false when the Newton iteration of the multilinear map fails
+template <int D>
 __device__ bool map_inv(const MeshDev& m, const int c[3], const double x[3], double xi[3]) {
-  const int nv = m.D == 3 ? 8 : 4;
+  const int nv = D == 3 ? 8 : 4;
   if (m.cartesian) {
-    const double* a = cell_vertex(m, c, 0);
-    const double* b = cell_vertex(m, c, nv - 1);
-    for (int d = 0; d < 3; d++) xi[d] = d < m.D ? (x[d] - a[d]) / (b[d] - a[d]) : 0.0;
+    const double* a = cell_vertex<D>(m, c, 0);
+    const double* b = cell_vertex<D>(m, c, nv - 1);
+    for (int d = 0; d < 3; d++) xi[d] = d < D ? (x[d] - a[d]) / (b[d] - a[d]) : 0.0;
     return true;
   }
-  for (int d = 0; d < 3; d++) xi[d] = d < m.D ? 0.5 : 0.0;
+  for (int d = 0; d < 3; d++) xi[d] = d < D ? 0.5 : 0.0;
+  double prev = 1e300;
   for (int it = 0; it < 100; it++) {
     double y[3], J[3][3], r[3], s[3];
-    map_fwd(m, c, xi, y);
-    for (int d = 0; d < 3; d++) r[d] = d < m.D ? y[d] - x[d] : 0.0;
-    jac(m, c, xi, J);
-    if (m.D == 2) {
+    map_fwd<D>(m, c, xi, y);
+    for (int d = 0; d < 3; d++) r[d] = d < D ? y[d] - x[d] : 0.0;
+    jac<D>(m, c, xi, J);
+    if (D == 2) {
       const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
       if (det == 0) return false;
       s[0] = (r[0] * J[1][1] - J[0][1] * r[1]) / det;
@@ -98,17 +104,21 @@ __device__ bool map_inv(const MeshDev& m, const int c[3], const double x[3], dou
       s[2] = (c02 * r[0] + (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r[1] + (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r[2]) / det;
     }
     double mx = 0;
-    for (int d = 0; d < m.D; d++) {
+    for (int d = 0; d < D; d++) {
       xi[d] -= s[d];
       mx = fmax(mx, fabs(s[d]));
     }
     if (mx < 1e-15) return true;
+    // converged to the rounding level of the residual: the step no longer shrinks
+    if (mx < 1e-12 && mx > 0.25 * prev) return true;
+    prev = mx;
     if (fabs(xi[0]) > 1e3) return false;
   }
   return true;
 }
 
 // neighbour across face f; -1 at a non-periodic boundary
+template <int D>
 __device__ int neighbor(const MeshDev& m, const int c[3], int f, bool* wrapped, int nc[3]) {
   const int a = f >> 1;
   nc[0] = c[0];
@@ -124,37 +134,39 @@ __device__ int neighbor(const MeshDev& m, const int c[3], int f, bool* wrapped, 
   return nc[0] + m.N[0] * (nc[1] + m.N[1] * nc[2]);
 }
 
+template <int D>
 __device__ void periodic_shift(const MeshDev& m, const int c[3], int f, double p[3]) {
   const int a = f >> 1;
   int lo[3] = {c[0], c[1], c[2]}, hi[3] = {c[0], c[1], c[2]};
   lo[a] = 0;
   hi[a] = m.N[a];
-  const double* vlo = vtx(m, lo[0], lo[1], m.D == 3 ? lo[2] : 0);
-  const double* vhi = vtx(m, hi[0], hi[1], m.D == 3 ? hi[2] : 0);
-  for (int d = 0; d < m.D; d++) {
+  const double* vlo = vtx(m, lo[0], lo[1], D == 3 ? lo[2] : 0);
+  const double* vhi = vtx(m, hi[0], hi[1], D == 3 ? hi[2] : 0);
+  for (int d = 0; d < D; d++) {
     const double off = vlo[d] - vhi[d];
     p[d] = (f & 1) ? p[d] + off : p[d] - off;
   }
 }
 
+template <int D>
 __device__ int crossed_without_mapping(const MeshDev& m, const int c[3], const double pin[3], const double pout[3],
                                        double pb[3]) {
   double lmin = 1;
   int imin = 0;
-  for (int f = 0; f < 2 * m.D; f++) {
+  for (int f = 0; f < 2 * D; f++) {
     const int a = f >> 1, side = f & 1;
     int fv[4], k = 0;
-    for (int v = 0; v < (m.D == 3 ? 8 : 4); v++)
+    for (int v = 0; v < (D == 3 ? 8 : 4); v++)
       if (((v >> a) & 1) == side) fv[k++] = v;
-    const double* v0 = cell_vertex(m, c, fv[0]);
-    const double* v1 = cell_vertex(m, c, fv[1]);
+    const double* v0 = cell_vertex<D>(m, c, fv[0]);
+    const double* v1 = cell_vertex<D>(m, c, fv[1]);
     double n[3];
-    if (m.D == 2) {
+    if (D == 2) {
       n[0] = v1[1] - v0[1];
       n[1] = -(v1[0] - v0[0]);
       n[2] = 0;
     } else {
-      const double* v2 = cell_vertex(m, c, fv[2]);
+      const double* v2 = cell_vertex<D>(m, c, fv[2]);
       const double d1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
       const double d2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
       n[0] = d1[1] * d2[2] - d1[2] * d2[1];
@@ -162,7 +174,7 @@ __device__ int crossed_without_mapping(const MeshDev& m, const int c[3], const d
       n[2] = d1[0] * d2[1] - d1[1] * d2[0];
     }
     double num = 0, den = 0;
-    for (int d = 0; d < m.D; d++) {
+    for (int d = 0; d < D; d++) {
       num += (pin[d] - v0[d]) * n[d];
       den += (pin[d] - pout[d]) * n[d];
     }
@@ -178,16 +190,17 @@ __device__ int crossed_without_mapping(const MeshDev& m, const int c[3], const d
   return imin;
 }
 
+template <int D>
 __device__ int crossed_first(const MeshDev& m, const int c[3], const double pin[3], const double pout[3], double pb[3]) {
   double pi[3], po[3];
-  if (!map_inv(m, c, pin, pi) || !map_inv(m, c, pout, po)) return crossed_without_mapping(m, c, pin, pout, pb);
-  for (int i = 0; i < m.D; i++) {
+  if (!map_inv<D>(m, c, pin, pi) || !map_inv<D>(m, c, pout, po)) return crossed_without_mapping<D>(m, c, pin, pout, pb);
+  for (int i = 0; i < D; i++) {
     clamp01(pi[i]);
     clamp01(po[i]);
   }
   int face = -1;
   double lambda = 100;
-  for (int a = 0; a < m.D; a++) {
+  for (int a = 0; a < D; a++) {
     double lam;
     if (po[a] < 0)
       lam = (0 - pi[a]) / (po[a] - pi[a]);
@@ -202,11 +215,11 @@ __device__ int crossed_first(const MeshDev& m, const int c[3], const double pin[
   }
   if (face < 0) return -1;
   double h[3] = {0, 0, 0};
-  for (int i = 0; i < m.D; i++) {
+  for (int i = 0; i < D; i++) {
     h[i] = pi[i] + (po[i] - pi[i]) * lambda;
     clamp01(h[i]);
   }
-  map_fwd(m, c, h, pb);
+  map_fwd<D>(m, c, h, pb);
   return face;
 }
 
@@ -216,45 +229,46 @@ __device__ int crossed_first(const MeshDev& m, const int c[3], const double pin[
 // at the hit point and continues for the remaining distance in the same cell;
 // *nhit counts the wall hits (an odd count reads the opposite direction) and *bias
 // accumulates 2 w (e . u_wall) / cs2 (rho = 1, destination direction e) per hit.
+template <int D>
 __device__ int locate(const MeshDev& m, int owner, const double x[3], const double disp[3], const double e[3],
                       double w, double cs2, double xi[3], int* nhit, double* bias) {
   double cur[3] = {x[0], x[1], x[2]};
   double dep[3] = {x[0] + disp[0], x[1] + disp[1], x[2] + disp[2]};
   int c[3];
-  cell_xyz(m, owner, c);
+  cell_xyz<D>(m, owner, c);
   int cell = owner;
   double sgn = 1.0;  // current direction = sgn * e
   for (int life = 1;; life++) {
     if (life > 50) return -1;
     double pb[3];
-    const int face = crossed_first(m, c, cur, dep, pb);
+    const int face = crossed_first<D>(m, c, cur, dep, pb);
     if (face < 0) break;
     bool wrapped;
     int nc[3];
-    const int nb = neighbor(m, c, face, &wrapped, nc);
+    const int nb = neighbor<D>(m, c, face, &wrapped, nc);
     if (nb < 0) {
       sgn = -sgn;
       double vel = 0, dist = 0;
-      for (int d = 0; d < m.D; d++) {
+      for (int d = 0; d < D; d++) {
         const double ed = sgn * e[d];
         vel += ed * ed;
         dist += (dep[d] - pb[d]) * (dep[d] - pb[d]);
       }
       vel = sqrt(vel);
       dist = sqrt(dist);
-      for (int d = 0; d < m.D; d++) {
+      for (int d = 0; d < D; d++) {
         cur[d] = pb[d];
         dep[d] = pb[d] - (sgn * e[d]) * dist / vel;
       }
       double exu = 0;
-      for (int d = 0; d < m.D; d++) exu += m.wall_u[face][d] * e[d];
+      for (int d = 0; d < D; d++) exu += m.wall_u[face][d] * e[d];
       *bias = *bias + 2 * w * 1.0 * exu / cs2;
       (*nhit)++;
       continue;
     }
     if (wrapped) {
-      periodic_shift(m, c, face, pb);
-      periodic_shift(m, c, face, dep);
+      periodic_shift<D>(m, c, face, pb);
+      periodic_shift<D>(m, c, face, dep);
     }
     for (int d = 0; d < 3; d++) cur[d] = pb[d];
     c[0] = nc[0];
@@ -262,19 +276,20 @@ __device__ int locate(const MeshDev& m, int owner, const double x[3], const doub
     c[2] = nc[2];
     cell = nb;
   }
-  if (!map_inv(m, c, dep, xi)) return -3;
-  for (int d = 0; d < m.D; d++) {
+  if (!map_inv<D>(m, c, dep, xi)) return -3;
+  for (int d = 0; d < D; d++) {
     clamp01(xi[d]);
     if (xi[d] < 0 || xi[d] > 1) return -3;
   }
-  if (m.D == 2) xi[2] = 0;
+  if (D == 2) xi[2] = 0;
   return cell;
 }
 
 // One thread per (streamed direction, DoF). dirw holds per streamed direction the
 // scaled velocity (3) and the weight; with walls, flipmask[i] collects bit alpha
 // for a reversed source direction and bit 0 for any hit, bias_out the added value.
-__global__ void k_assemble_general(MeshDev m, int64_t n, int Qm1, const double* __restrict__ coords,
+template <int D>
+__global__ __launch_bounds__(128) void k_assemble_general(MeshDev m, int64_t n, int Qm1, const double* __restrict__ coords,
                                    const int* __restrict__ owner, const double* __restrict__ disp /* Qm1 x 3 */,
                                    const double* __restrict__ dirw /* Qm1 x 4 */, double cs2,
                                    int* __restrict__ cell_out, double* __restrict__ xi_out,
@@ -290,9 +305,9 @@ __global__ void k_assemble_general(MeshDev m, int64_t n, int Qm1, const double* 
   double xi[3] = {0, 0, 0};
   int nhit = 0;
   double bias = 0;
-  const int c = locate(m, owner[i], x, dp, e, dirw[4 * a + 3], cs2, xi, &nhit, &bias);
+  const int c = locate<D>(m, owner[i], x, dp, e, dirw[4 * a + 3], cs2, xi, &nhit, &bias);
   cell_out[t] = c;
-  for (int d = 0; d < m.D; d++) xi_out[t * m.D + d] = xi[d];
+  for (int d = 0; d < D; d++) xi_out[t * D + d] = xi[d];
   if (flipmask) {
     bias_out[t] = bias;
     // an identity row (life-time limit) keeps direction alpha; its hits still add
@@ -393,9 +408,14 @@ int nat_launch_assemble_general(nat_ctx* ctx, const double* d_disp, const double
   m.vert = ctx->d_vert;
   const int64_t total = ctx->n_dofs * (ctx->Q - 1);
   const int bs = 128;
-  hipLaunchKernelGGL(nat::k_assemble_general, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, ctx->stream, m,
-                     ctx->n_dofs, ctx->Q - 1, ctx->d_coords, ctx->d_owner, d_disp, d_dirw, ctx->st.cs2, d_cell, d_xi,
-                     d_flipmask, d_bias, ctx->d_status);
+  if (ctx->D == 3)
+    hipLaunchKernelGGL(nat::k_assemble_general<3>, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
+                       m, ctx->n_dofs, ctx->Q - 1, ctx->d_coords, ctx->d_owner, d_disp, d_dirw, ctx->st.cs2, d_cell,
+                       d_xi, d_flipmask, d_bias, ctx->d_status);
+  else
+    hipLaunchKernelGGL(nat::k_assemble_general<2>, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
+                       m, ctx->n_dofs, ctx->Q - 1, ctx->d_coords, ctx->d_owner, d_disp, d_dirw, ctx->st.cs2, d_cell,
+                       d_xi, d_flipmask, d_bias, ctx->d_status);
   NAT_HIP(hipGetLastError());
   return 0;
 }

@@ -204,6 +204,30 @@ bool is_uniform_cartesian(const nat_ctx* c) {
   return true;
 }
 
+// every cell an axis-aligned box (what MappingCartesian assumes)
+bool is_axis_aligned(const nat_ctx* c) {
+  const int nv = c->D == 3 ? 8 : 4;
+  const int nz = c->D == 3 ? c->desc.ncells[2] : 1;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < c->desc.ncells[1]; j++)
+      for (int i = 0; i < c->desc.ncells[0]; i++) {
+        int cc[3] = {i, j, k};
+        double lo[3], hi[3], X[3];
+        cell_vertex(c, cc, 0, lo);
+        cell_vertex(c, cc, nv - 1, hi);
+        for (int v = 0; v < nv; v++) {
+          cell_vertex(c, cc, v, X);
+          for (int a = 0; a < c->D; a++) {
+            const double want = ((v >> a) & 1) ? hi[a] : lo[a];
+            if (std::fabs(X[a] - want) > 1e-12 * (std::fabs(hi[a] - lo[a]) + 1e-300)) return false;
+          }
+        }
+        for (int a = 0; a < c->D; a++)
+          if (!(hi[a] > lo[a])) return false;
+      }
+  return true;
+}
+
 template <typename T>
 int alloc(nat_ctx* ctx, T** p, size_t count) {
   *p = nullptr;
@@ -699,6 +723,42 @@ int nat_node_coords(nat_ctx* ctx, double* xyz) {
   }
   std::memcpy(xyz, x.data(), x.size() * sizeof(double));
   return 0;
+}
+
+int nat_set_vertices(nat_ctx* ctx, const double* vertices) {
+  if (!vertices) return fail(ctx, NAT_ERR_ARG, "vertices required");
+  const nat_desc& d = ctx->desc;
+  const int nz = d.dim == 3 ? d.ncells[2] + 1 : 1;
+  const size_t nv = (size_t)(d.ncells[0] + 1) * (d.ncells[1] + 1) * nz;
+  std::vector<double> old = ctx->verts;
+  for (size_t i = 0; i < nv; i++)
+    for (int a = 0; a < d.dim; a++) ctx->verts[3 * i + a] = vertices[i * d.dim + a];
+  if (d.cartesian && !is_axis_aligned(ctx)) {
+    ctx->verts = old;
+    return fail(ctx, NAT_ERR_ARG, "a context created with cartesian = 1 needs axis-aligned cells");
+  }
+  if (d.scheme == NAT_SL && ctx->backend == NAT_BACKEND_STRUCTURED && !is_uniform_cartesian(ctx)) {
+    ctx->verts = old;
+    return fail(ctx, NAT_ERR_UNSUPPORTED, "the structured backend needs a uniform Cartesian mesh");
+  }
+  NAT_HIP(hipMemcpy(ctx->d_vert, ctx->verts.data(), ctx->verts.size() * sizeof(double), hipMemcpyHostToDevice));
+  ctx->h_min = min_vertex_distance(ctx);
+  if (d.scheme != NAT_SL) return 0;  // SEDG evaluates the geometry from the vertices in every stage
+  std::vector<double> xyz;
+  std::vector<int> owner;
+  feq_points(ctx, xyz, owner);
+  NAT_HIP(hipMemcpy(ctx->d_coords, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (ctx->backend == NAT_BACKEND_STRUCTURED) {
+    if (ctx->ax_w) hipFree(ctx->ax_w);
+    if (ctx->ax_off) hipFree(ctx->ax_off);
+    ctx->ax_w = nullptr;
+    ctx->ax_off = nullptr;
+    bool ok = false;
+    TRY(setup_structured_tables(ctx, &ok));
+    if (!ok) return fail(ctx, NAT_ERR_UNSUPPORTED, "moved mesh gives a non-separable operator for the structured backend");
+    return 0;
+  }
+  return assemble_general(ctx);
 }
 
 int nat_set_stabilizer(nat_ctx* ctx, int on) {

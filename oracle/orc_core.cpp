@@ -239,6 +239,7 @@ bool Mesh::inverse_map(int cell, const double x[3], double xi[3]) const {
   }
   // Newton iteration on the multilinear map, started at the cell centre.
   for (int d = 0; d < 3; d++) xi[d] = d < D ? 0.5 : 0.0;
+  double prev = 1e300;
   for (int it = 0; it < 100; it++) {
     double y[3], J[3][3], r[3], s[3];
     map(cell, xi, y);
@@ -251,6 +252,9 @@ bool Mesh::inverse_map(int cell, const double x[3], double xi[3]) const {
       m = std::max(m, std::fabs(s[d]));
     }
     if (m < 1e-15) return true;
+    // converged to the rounding level of the residual: the step no longer shrinks
+    if (m < 1e-12 && m > 0.25 * prev) return true;
+    prev = m;
     if (std::fabs(xi[0]) > 1e3) return false;
   }
   return true;
