@@ -156,6 +156,41 @@ int nat_sums(nat_ctx* ctx, double* mass, double* momentum /* dim */, double* eki
 
 int nat_sync(nat_ctx* ctx);
 
+/* ---- Output and restart (host side, from fields downloaded through this API) ---- */
+
+/* One row of the reference's results table: SolverStats::update/printNewLine
+ * (solver/SolverStats.h:102-175) with PhysicalProperties (solver/PhysicalProperties.cpp:29-420):
+ * max |u| is the largest component-wise maximum, enstrophy is int (dv/dx - du/dy)^2 in 2D and
+ * int |grad u|_F^2 in 3D on the GLL quadrature (enstrophy_sq the integral of its square),
+ * entropy is -sum_cells (sum_q JxW_q) sum_i sum_j f_j ln(f_j / w_j), pressures are
+ * (rho - 1) c_s^2, mean(ux) the volume-weighted mean of the cell DoF averages. Residuals are
+ * formed at iterations divisible by 10 against the fields of the previous such call, once 100
+ * iterations have passed since the first (SolverStats::calculateResiduals); otherwise the
+ * last values are repeated. */
+typedef struct nat_stats_t {
+  int64_t iteration;
+  double time, max_u, mass, kin_e, enstrophy, entropy, max_p, min_p, res_rho, res_u, mean_ux, enstrophy_sq;
+} nat_stats_t;
+int nat_stats(nat_ctx* ctx, int64_t iteration, nat_stats_t* out);
+/* Appends the row to the table file (columns as results_table.txt); truncate = 1 starts a
+ * new file with the header line. */
+int nat_write_stats(nat_ctx* ctx, const char* path, int64_t iteration, int truncate);
+
+/* CFDSolver::output solution files (solver/CFDSolver.cpp:1001-1069): dir/t_0.<it>.vtu with one
+ * patch per cell sampled like DataOut::build_patches(p + 1) (p = 1: one sub-cell), point data
+ * rho, ux, uy[, uz][, T] and cell data subdomain, plus the dir/t_0.<it>.pvtu record. */
+int nat_write_vtu(nat_ctx* ctx, const char* dir, int64_t iteration);
+
+/* Checkpoint::write / load (solver/Checkpoint.cpp:21-140): dir/checkpoint_<it>.stat holds the
+ * iteration, time, stencil scaling and FE order, one per line, as the reference writes it;
+ * dir/checkpoint_<it>.data holds the FP64 populations in DoF order (the reference stores a
+ * p4est/deal.II serialization there). Compressible contexts add checkpointG_<it>.* for g.
+ * Reading with iteration == 1 picks the most recent checkpoint, as in the reference; it sets
+ * the context time and returns the stored iteration. Another order or scaling returns
+ * NAT_ERR_UNSUPPORTED (the reference interpolates and rescales). */
+int nat_checkpoint_write(nat_ctx* ctx, const char* dir, int64_t iteration);
+int nat_checkpoint_read(nat_ctx* ctx, const char* dir, int64_t iteration, int64_t* iteration_out);
+
 /* ------------------------------------------------------------------------
  * Request server. No counterpart in the reference (one solve per process):
  * continuous batching of independent solves on one GPU. Requests are admitted

@@ -43,12 +43,19 @@ class _Desc(ctypes.Structure):
     ]
 
 
+class _Stats(ctypes.Structure):
+    _fields_ = [("iteration", ctypes.c_int64)] + [(nm, ctypes.c_double) for nm in (
+        "time", "max_u", "mass", "kin_e", "enstrophy", "entropy", "max_p", "min_p", "res_rho", "res_u", "mean_ux",
+        "enstrophy_sq")]
+
+
 EXPORTS = [
     "nat_create", "nat_destroy", "nat_last_error", "nat_n_dofs", "nat_n_stored_nodes", "nat_Q", "nat_dt",
     "nat_tau", "nat_time", "nat_backend", "nat_node_coords", "nat_set_dt", "nat_set_vertices", "nat_set_stabilizer",
     "nat_init_equilibrium",
     "nat_upload_f", "nat_download_f", "nat_upload_g", "nat_download_g", "nat_collide", "nat_stream",
     "nat_step", "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sync",
+    "nat_stats", "nat_write_stats", "nat_write_vtu", "nat_checkpoint_write", "nat_checkpoint_read",
 ]
 
 _lib = None
@@ -83,6 +90,16 @@ def lib():
     L.nat_node_coords.argtypes = [vp, _dp]
     L.nat_set_dt.restype = ctypes.c_int
     L.nat_set_dt.argtypes = [vp, ctypes.c_double]
+    L.nat_stats.restype = ctypes.c_int
+    L.nat_stats.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(_Stats)]
+    L.nat_write_stats.restype = ctypes.c_int
+    L.nat_write_stats.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64, ctypes.c_int]
+    L.nat_write_vtu.restype = ctypes.c_int
+    L.nat_write_vtu.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
+    L.nat_checkpoint_write.restype = ctypes.c_int
+    L.nat_checkpoint_write.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64]
+    L.nat_checkpoint_read.restype = ctypes.c_int
+    L.nat_checkpoint_read.argtypes = [vp, ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     L.nat_set_vertices.restype = ctypes.c_int
     L.nat_set_vertices.argtypes = [vp, _dp]
     L.nat_set_stabilizer.restype = ctypes.c_int
@@ -292,6 +309,27 @@ class Solver:
         e = np.zeros(1)
         self._check(lib().nat_sums(self._h, _ptr(m), _ptr(mom), _ptr(e)))
         return float(m[0]), mom[:self.D].copy(), float(e[0])
+
+    def stats(self, iteration):
+        """One results-table row as a dict (column names of the reference's table)."""
+        st = _Stats()
+        self._check(lib().nat_stats(self._h, int(iteration), ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _Stats._fields_}
+
+    def write_stats(self, path, iteration, truncate=False):
+        self._check(lib().nat_write_stats(self._h, os.fsencode(path), int(iteration), int(truncate)))
+
+    def write_vtu(self, directory, iteration):
+        self._check(lib().nat_write_vtu(self._h, os.fsencode(directory), int(iteration)))
+
+    def checkpoint_write(self, directory, iteration):
+        self._check(lib().nat_checkpoint_write(self._h, os.fsencode(directory), int(iteration)))
+
+    def checkpoint_read(self, directory, iteration=1):
+        """Load checkpoint `iteration` (1: the most recent); returns the stored iteration."""
+        it = ctypes.c_int64(0)
+        self._check(lib().nat_checkpoint_read(self._h, os.fsencode(directory), int(iteration), ctypes.byref(it)))
+        return it.value
 
     def sync(self):
         self._check(lib().nat_sync(self._h))

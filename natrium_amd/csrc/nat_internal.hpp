@@ -152,6 +152,11 @@ struct nat_ctx {
   int64_t n_wall = 0;
   bool has_walls = false;
 
+  // residuals of the results table: fields of the previous 10th-iteration call
+  std::vector<double> res_rho, res_u;
+  int64_t res_first = 0;
+  double res_rho_val = 0, res_u_val = 0;
+
   // pseudo-entropic stabilizer as a data processor after each loop step's collision
   // (D2Q9 and D3Q19 only, as in the reference)
   bool stabilizer = false;
