@@ -264,15 +264,17 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   const int org[3] = {(bid % a.tiles[0]) * TX, ((bid / a.tiles[0]) % a.tiles[1]) * TY,
                       (bid / (a.tiles[0] * a.tiles[1])) * TZ};
   const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
+  const unsigned ox0 = tid % TX, oy0 = (tid / TX) % TY, oz0 = tid / (TX * TY);
+  const bool active = (org[0] + (int)ox0 < nx) && (org[1] + (int)oy0 < ny) && (org[2] + (int)oz0 < nz);
+  const int node = (org[0] + ox0) + nx * ((org[1] + oy0) + ny * (org[2] + oz0));
+  T fs[Q];
+  fs[0] = active ? a.fin[node] : T(0);
   const int astr[3] = {1, nx, nx * ny};
   for (int i = tid; i < 3 * TW; i += NT) {
     const int ax = i / TW, k = i % TW;
     const int n = a.n[ax];
     gt[ax][k] = (ax < D) ? (((org[ax] - P + k) % n + n) % n) * astr[ax] * (int)sizeof(T) : 0;
   }
-  const unsigned ox0 = tid % TX, oy0 = (tid / TX) % TY, oz0 = tid / (TX * TY);
-  const bool active = (org[0] + (int)ox0 < nx) && (org[1] + (int)oy0 < ny) && (org[2] + (int)oz0 < nz);
-  const int node = (org[0] + ox0) + nx * ((org[1] + oy0) + ny * (org[2] + oz0));
 
   double W0[SYM ? P : 1][P + 1];
   if constexpr (SYM) {
@@ -282,14 +284,14 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
       for (int m = 0; m <= P; m++) W0[j][m] = wtab[j * (MAXP + 1) + m];
   }
 
-  T fs[Q];
-  fs[0] = active ? a.fin[node] : T(0);
   // first-pass inputs of the next PF groups (register slot sets, rotating by group)
   constexpr int PF = (sizeof(T) == 4) ? NAT_PREFETCH32 : NAT_PREFETCH64;
   T xin[PF][B][KI][P + 1];
   int xo0[PF][B][KI];         // output base of the first-pass item, computed with its loads
   unsigned xsc[PF][B][KI];    // its source cell along the pass axis
-  __syncthreads();
+  // the offset tables are LDS: wait for LDS only, so the rest population's global load
+  // stays in flight (a full __syncthreads would wait for it here)
+  lds_barrier();
 
   // raw buffer loads: 32-bit byte offsets against a per-direction resource (range checked)
   auto load_t = [](__amdgpu_buffer_rsrc_t r, int off) -> T {

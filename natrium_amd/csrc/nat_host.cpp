@@ -541,6 +541,15 @@ int assemble_general(nat_ctx* ctx) {
   return check_status(ctx);
 }
 
+void nat_drop_graphs(nat_ctx* ctx) {
+  for (int b = 0; b < 2; b++) {
+    if (ctx->gexec[b]) hipGraphExecDestroy(ctx->gexec[b]);
+    if (ctx->graph[b]) hipGraphDestroy(ctx->graph[b]);
+    ctx->gexec[b] = nullptr;
+    ctx->graph[b] = nullptr;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -560,6 +569,7 @@ int nat_backend(const nat_ctx* ctx) { return ctx->backend; }
 void nat_destroy(nat_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  nat_drop_graphs(ctx);
   void* bufs[] = {ctx->f[0],     ctx->f[1],        ctx->g[0],     ctx->g[1],       ctx->rho,       ctx->u,
                   ctx->T,        ctx->mask,        ctx->d_vert,   ctx->d_gll,      ctx->d_coords,  ctx->d_owner,
                   ctx->d_status, ctx->d_sums,      ctx->dep_cell, ctx->dep_xi64,   ctx->dep_off32, ctx->dep_code32, ctx->ax_w,
@@ -727,6 +737,7 @@ int nat_node_coords(nat_ctx* ctx, double* xyz) {
 
 int nat_set_vertices(nat_ctx* ctx, const double* vertices) {
   if (!vertices) return fail(ctx, NAT_ERR_ARG, "vertices required");
+  nat_drop_graphs(ctx);
   const nat_desc& d = ctx->desc;
   const int nz = d.dim == 3 ? d.ncells[2] + 1 : 1;
   const size_t nv = (size_t)(d.ncells[0] + 1) * (d.ncells[1] + 1) * nz;
@@ -762,11 +773,13 @@ int nat_set_vertices(nat_ctx* ctx, const double* vertices) {
 }
 
 int nat_set_stabilizer(nat_ctx* ctx, int on) {
+  nat_drop_graphs(ctx);
   ctx->stabilizer = on != 0;
   return 0;
 }
 
 int nat_set_dt(nat_ctx* ctx, double dt) {
+  nat_drop_graphs(ctx);
   if (dt > 0) {
     ctx->dt = dt;
     ctx->tau = ctx->desc.viscosity / (ctx->dt * ctx->st.cs2) + 0.5;
@@ -938,10 +951,10 @@ int nat_collide(nat_ctx* ctx) {
   return check_status(ctx);
 }
 
-static int step_launch(nat_ctx* ctx, int nsteps) {
+static int step_launch(nat_ctx* ctx, int nsteps, bool macro_last = true) {
   if (!ctx->assembled) return fail(ctx, NAT_ERR_STATE, "operator not assembled");
   for (int s = 0; s < nsteps; s++) {
-    const int wm = (s == nsteps - 1) ? 1 : 0;
+    const int wm = (macro_last && s == nsteps - 1) ? 1 : 0;
     if (ctx->desc.compressible) {
       TRY(nat_launch_sl_comp(ctx, wm));
     } else if (ctx->desc.scheme == NAT_SL) {
@@ -957,8 +970,44 @@ static int step_launch(nat_ctx* ctx, int nsteps) {
   return 0;
 }
 
+// Long runs of steps go to the device as replays of a captured HIP graph of
+// kGraphSteps fused steps: one submission per chunk instead of one per kernel, so
+// dependent launches follow each other without host launch latency in between. An
+// even chunk returns the ping-pong buffers to where it started; one graph is kept
+// per starting buffer. Anything baked into the kernel arguments (dt, tables,
+// stabilizer) drops the graphs when it changes.
+static constexpr int kGraphSteps = 32;
+
+static int step_enqueue(nat_ctx* ctx, int nsteps) {
+  int left = nsteps;
+  // the last step writes the macroscopic fields and always runs as a plain launch
+  while (ctx->use_graphs && left - 1 >= kGraphSteps) {
+    const int par = ctx->cur;
+    if (!ctx->gexec[par]) {
+      const double t = ctx->time;
+      NAT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = step_launch(ctx, kGraphSteps, false);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      ctx->time = t;
+      ctx->cur = par;
+      if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+      }
+      NAT_HIP(e);
+      ctx->graph[par] = g;
+      NAT_HIP(hipGraphInstantiate(&ctx->gexec[par], g, nullptr, nullptr, 0));
+    }
+    NAT_HIP(hipGraphLaunch(ctx->gexec[par], ctx->stream));
+    ctx->time += kGraphSteps * ctx->dt;
+    left -= kGraphSteps;
+  }
+  return step_launch(ctx, left);
+}
+
 int nat_step(nat_ctx* ctx, int nsteps) {
-  TRY(step_launch(ctx, nsteps));
+  TRY(step_enqueue(ctx, nsteps));
   return check_status(ctx);
 }
 
@@ -967,7 +1016,7 @@ int nat_step_timed(nat_ctx* ctx, int nsteps, double* elapsed_ms) {
   NAT_HIP(hipEventCreate(&a));
   NAT_HIP(hipEventCreate(&b));
   NAT_HIP(hipEventRecord(a, ctx->stream));
-  int rc = step_launch(ctx, nsteps);
+  int rc = step_enqueue(ctx, nsteps);
   if (rc == 0) {
     NAT_HIP(hipEventRecord(b, ctx->stream));
     NAT_HIP(hipEventSynchronize(b));

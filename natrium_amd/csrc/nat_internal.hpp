@@ -152,6 +152,11 @@ struct nat_ctx {
   int64_t n_wall = 0;
   bool has_walls = false;
 
+  // captured step chunks, one per starting ping-pong buffer
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  bool use_graphs = true;
+
   // residuals of the results table: fields of the previous 10th-iteration call
   std::vector<double> res_rho, res_u;
   int64_t res_first = 0;
