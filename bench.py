@@ -107,6 +107,7 @@ def extra_configs(nat, steps_sl=100, steps_err=500, cpu=False):
         RK5 (884,736 DG nodes).
     C4: unit square, 128^2 quads, p=2, D2Q25H compressible f/g, Riemann quadrant states
         (seed 7), velocity-neq bounce-back walls at rest on all four sides, errors after 2000 steps.
+    Large C2: the same 3D TGV on 48^3 cells (7.08M stored nodes, beyond the MALL), 40 timed steps.
     Reassembly: the C3 mesh moved to a second perturbation (amplitude 0.12 h, seed 77) and the SL
         operator rebuilt (FP64 departure walk through the Q1 map, 389,017 DoFs x 18 directions):
         wall time of nat_set_vertices on the GPU and, with --cpu-assembly, of the oracle's
@@ -118,6 +119,26 @@ def extra_configs(nat, steps_sl=100, steps_err=500, cpu=False):
                                       riemann_quadrant_fields, riemann_states)
 
     out = {}
+    # the C2 kernel on a working set beyond the 256 MB MALL: 48^3 cells, p = 4 (192^3 = 7.08M nodes,
+    # 1.53 GB of FP32 populations per buffer pair, 3.06 GB in FP64); one tile per block as in C2,
+    # 27 tiles per CU
+    big = tgv3d_problem(N=48)
+    res = {}
+    for prec in ("fp32", "fp64"):
+        sol = make_solver(nat, big, prec)
+        init_tgv(sol, big)
+        sol.step(4)
+        ms = sol.step_timed(40)
+        n = sol.n_stored
+        res[prec + "_glups"] = round(n * 40 / (ms / 1e3) / 1e9, 4)
+        res[prec + "_us_per_step"] = round(ms * 1e3 / 40, 2)
+        if prec == "fp32":
+            res["hbm_frac_fp32"] = round(n * 2 * 27 * 4 / (ms / 1e3 / 40) / 1e9 / HBM_PEAK_GBS, 4)
+        del sol
+    res["stored_nodes"] = 192 ** 3
+    res["fp32_over_fp64"] = round(res["fp32_glups"] / res["fp64_glups"], 3)
+    out["tgv3d_48c_p4_d3q27"] = res
+
     cs, s = lbm_parameters(1.0, 0.05)
     L = 2 * math.pi
     v3 = perturbed_vertices(3, [24] * 3, [0] * 3, [L] * 3, amplitude=0.1, seed=1234)
