@@ -192,43 +192,57 @@ __device__ __forceinline__ void lag_all(const double (&x)[MAXP + 1], const doubl
 }
 
 // fused SL stream (f and g, general backend records) + compressible collision
+// Streamed f and g of direction al at DoF i (al uniform across the wave), with the
+// FP32 shift added back. One set of FP64 weights serves f and g.
 template <int Q, int P, typename T>
-__global__ __launch_bounds__(256) void k_sl_comp(CompArgs<T> a, const double* __restrict__ tab, int write_macro) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  double fl[Q], gl[Q];
-  fl[0] = (double)a.fin[i] + tab[CT_W] * a.rho0;
-  gl[0] = (double)a.gin[i] + tab[CT_W] * a.gshift;
+__device__ __forceinline__ void comp_stream_dir(const CompArgs<T>& a, const double* __restrict__ tab, int64_t i,
+                                                int al, int widx, unsigned wmask, double& fo, double& go) {
   const int n0 = a.nfe[0];
-  int widx = -1;
-  unsigned wmask = 0;
-  if (a.wall_idx) {
-    widx = a.wall_idx[i];
-    if (widx >= 0) wmask = a.wall_flip[widx];
-  }
+  const int64_t r = (int64_t)(al - 1) * a.n + i;
+  const int sd = ((wmask >> al) & 1u) ? a.opp[al] : al;
+  const T* fsrc = a.fin + (int64_t)sd * a.stride;
+  const T* gsrc = a.gin + (int64_t)sd * a.stride;
+  const double bias = widx >= 0 ? a.wall_bias[(int64_t)widx * Q + al] : 0.0;
+  const int c = a.cell[r];
+  double af, ag;
+  if (c < 0) {
+    af = (double)a.fin[(int64_t)al * a.stride + i];
+    ag = (double)a.gin[(int64_t)al * a.stride + i];
+  } else {
+    double lx[P + 1], ly[P + 1];
+    lag_all<P>(a.gll, a.inv_den, a.xi[2 * r + 0], lx);
+    lag_all<P>(a.gll, a.inv_den, a.xi[2 * r + 1], ly);
+    const int cx = c % a.N[0], cy = c / a.N[0];
+    const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P);
+    af = 0;
+    ag = 0;
+    double mnx = fabs(lx[0]), mny = fabs(ly[0]);
 #pragma unroll
-  for (int al = 1; al < Q; al++) {
-    const int64_t r = (int64_t)(al - 1) * a.n + i;
-    // velocity-neq bounce-back: read the opposite direction at the reflected point; the
-    // wall value is added to f only (g is streamed by the same matrix with no boundary
-    // values, CompressibleCFDSolver.h:279-314)
-    const int sd = ((wmask >> al) & 1u) ? a.opp[al] : al;
-    const T* fsrc = a.fin + (int64_t)sd * a.stride;
-    const T* gsrc = a.gin + (int64_t)sd * a.stride;
-    const double bias = widx >= 0 ? a.wall_bias[(int64_t)widx * Q + al] : 0.0;
-    const int c = a.cell[r];
-    double af, ag;
-    if (c < 0) {
-      af = (double)a.fin[(int64_t)al * a.stride + i];
-      ag = (double)a.gin[(int64_t)al * a.stride + i];
+    for (int m = 1; m <= P; m++) {
+      mnx = fmin(mnx, fabs(lx[m]));
+      mny = fmin(mny, fabs(ly[m]));
+    }
+    if (mnx * mny >= 1e-10) {
+      // no product weight reaches the cut-off: every term is taken, with the loads of
+      // the whole stencil issued together (same terms in the same order)
+      double vf[P + 1][P + 1], vg[P + 1][P + 1];
+#pragma unroll
+      for (int my = 0; my <= P; my++)
+#pragma unroll
+        for (int mx = 0; mx <= P; mx++) {
+          const int64_t k = base + mx + (int64_t)n0 * my;
+          vf[my][mx] = (double)fsrc[k];
+          vg[my][mx] = (double)gsrc[k];
+        }
+#pragma unroll
+      for (int my = 0; my <= P; my++)
+#pragma unroll
+        for (int mx = 0; mx <= P; mx++) {
+          const double w = lx[mx] * ly[my];
+          af += w * vf[my][mx];
+          ag += w * vg[my][mx];
+        }
     } else {
-      double lx[P + 1], ly[P + 1];
-      lag_all<P>(a.gll, a.inv_den, a.xi[2 * r + 0], lx);
-      lag_all<P>(a.gll, a.inv_den, a.xi[2 * r + 1], ly);
-      const int cx = c % a.N[0], cy = c / a.N[0];
-      const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P);
-      af = 0;
-      ag = 0;
 #pragma unroll
       for (int my = 0; my <= P; my++)
 #pragma unroll
@@ -241,10 +255,50 @@ __global__ __launch_bounds__(256) void k_sl_comp(CompArgs<T> a, const double* __
           }
         }
     }
-    af = af + bias;
-    // shifted storage: interpolation preserves constants, so the shift is added back here
-    fl[al] = af + tab[al * CTAB + CT_W] * a.rho0;
-    gl[al] = ag + tab[al * CTAB + CT_W] * a.gshift;
+  }
+  af = af + bias;
+  fo = af + tab[al * CTAB + CT_W] * a.rho0;
+  go = ag + tab[al * CTAB + CT_W] * a.gshift;
+}
+
+// Fused SL stream of f and g + compressible collision. The gathers are spread over the
+// waves of a block: 64 DoFs per block, CW waves each streaming every CW-th direction of
+// those DoFs (one thread per DoF would give a 66k-DoF grid only four waves per CU to hide
+// its gather latency). The streamed values meet in LDS and the first wave collides.
+constexpr int CW = 8;
+template <int Q, int P, typename T>
+__global__ __launch_bounds__(CW * 64) void k_sl_comp_wide(CompArgs<T> a, const double* __restrict__ tab,
+                                                          int write_macro) {
+  __shared__ double sf[Q][64], sg[Q][64];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  const bool live = i < a.n;
+  int widx = -1;
+  unsigned wmask = 0;
+  if (live && a.wall_idx) {
+    widx = a.wall_idx[i];
+    if (widx >= 0) wmask = a.wall_flip[widx];
+  }
+  if (live) {
+    if (w == 0) {
+      sf[0][lane] = (double)a.fin[i] + tab[CT_W] * a.rho0;
+      sg[0][lane] = (double)a.gin[i] + tab[CT_W] * a.gshift;
+    }
+    for (int al = 1 + w; al < Q; al += CW) {
+      double fo, go;
+      comp_stream_dir<Q, P, T>(a, tab, i, al, widx, wmask, fo, go);
+      sf[al][lane] = fo;
+      sg[al][lane] = go;
+    }
+  }
+  __syncthreads();
+  if (w != 0 || !live) return;
+  double fl[Q], gl[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    fl[q] = sf[q][lane];
+    gl[q] = sg[q][lane];
   }
   comp_collide<Q, T>(fl, gl, a, tab, i, write_macro != 0);
 #pragma unroll
@@ -330,13 +384,14 @@ static int step_comp(nat_ctx* ctx, int write_macro) {
   CompArgs<T> a;
   const int src = ctx->cur, dst = 1 - ctx->cur;
   fill_comp<T>(ctx, a, src, dst);
-  const unsigned grid = (unsigned)((a.n + 255) / 256);
+  const unsigned grid = (unsigned)((a.n + 63) / 64);
   const double* tab = ctx->comp_tab;
+  const dim3 blk(CW * 64);
   switch (ctx->p) {
-    case 1: hipLaunchKernelGGL((k_sl_comp<25, 1, T>), dim3(grid), dim3(256), 0, ctx->stream, a, tab, write_macro); break;
-    case 2: hipLaunchKernelGGL((k_sl_comp<25, 2, T>), dim3(grid), dim3(256), 0, ctx->stream, a, tab, write_macro); break;
-    case 3: hipLaunchKernelGGL((k_sl_comp<25, 3, T>), dim3(grid), dim3(256), 0, ctx->stream, a, tab, write_macro); break;
-    case 4: hipLaunchKernelGGL((k_sl_comp<25, 4, T>), dim3(grid), dim3(256), 0, ctx->stream, a, tab, write_macro); break;
+    case 1: hipLaunchKernelGGL((k_sl_comp_wide<25, 1, T>), dim3(grid), blk, 0, ctx->stream, a, tab, write_macro); break;
+    case 2: hipLaunchKernelGGL((k_sl_comp_wide<25, 2, T>), dim3(grid), blk, 0, ctx->stream, a, tab, write_macro); break;
+    case 3: hipLaunchKernelGGL((k_sl_comp_wide<25, 3, T>), dim3(grid), blk, 0, ctx->stream, a, tab, write_macro); break;
+    case 4: hipLaunchKernelGGL((k_sl_comp_wide<25, 4, T>), dim3(grid), blk, 0, ctx->stream, a, tab, write_macro); break;
     default:
       ctx->err = "compressible SL kernel instantiated for p <= 4";
       return NAT_ERR_UNSUPPORTED;
