@@ -94,29 +94,32 @@ struct QuarticNode {
   }
 };
 
-// Compressible collision of one node; fl, gl unshifted FP64, relaxed in place.
-template <int Q, typename T>
-__device__ __forceinline__ void comp_collide(double (&fl)[Q], double (&gl)[Q], const CompArgs<T>& a,
-                                             const double* __restrict__ tab, int64_t node, bool write_macro) {
+// Compressible collision of one node. fl(q), gl(q) give the unshifted FP64
+// populations; out(q, f_new, g_new) receives the relaxed ones. The accessors let the
+// values live in registers or in LDS (the fused step keeps them in LDS, which leaves
+// the registers to the gathers of the block's other waves).
+template <int Q, typename T, typename FA, typename GA, typename OUT>
+__device__ __forceinline__ void comp_collide_acc(FA fl, GA gl, OUT out, const CompArgs<T>& a,
+                                                 const double* __restrict__ tab, int64_t node, bool write_macro) {
   constexpr int D = 2;
   const double cs2 = a.cs2;
   const double C_v = 1. / (a.gamma - 1.0);
   double rho = 0;
-#pragma unroll
-  for (int q = 0; q < Q; q++) rho += fl[q];
+#pragma unroll 1
+  for (int q = 0; q < Q; q++) rho += fl(q);
   if (!(rho >= 1e-10)) atomicOr(a.status, 1);
   double j0 = 0, j1 = 0;
-#pragma unroll
+#pragma unroll 1
   for (int q = 0; q < Q; q++) {
-    j0 += tab[q * CTAB + CT_E0] * fl[q];
-    j1 += tab[q * CTAB + CT_E1] * fl[q];
+    j0 += tab[q * CTAB + CT_E0] * fl(q);
+    j1 += tab[q * CTAB + CT_E1] * fl(q);
   }
   const double u0 = j0 * (1.0 / rho), u1 = j1 * (1.0 / rho);
   double tsum = 0;
-#pragma unroll
+#pragma unroll 1
   for (int q = 0; q < Q; q++) {
     const double d0 = tab[q * CTAB + CT_E0] - u0, d1 = tab[q * CTAB + CT_E1] - u1;
-    tsum += (d0 * d0 + d1 * d1) * fl[q] / cs2 + gl[q];
+    tsum += (d0 * d0 + d1 * d1) * fl(q) / cs2 + gl(q);
   }
   const double temp = tsum * 0.5 / (rho * C_v);
   QuarticNode qn;
@@ -134,14 +137,14 @@ __device__ __forceinline__ void comp_collide(double (&fl)[Q], double (&gl)[Q], c
   double Qt[2][2][2] = {{{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}};
   double Fg[2] = {0, 0};
   double kn = 0;
-#pragma unroll
+#pragma unroll 1
   for (int q = 0; q < Q; q++) {
     const double* t = tab + q * CTAB;
     const double fe = qn.feq(t, cs2);
-    const double fneq = fl[q] - fe;
+    const double fneq = fl(q) - fe;
     kn += fabs(fneq) / t[CT_W];
     if (a.prandtl_set) {
-      const double gneq = gl[q] - fe * gfac;
+      const double gneq = gl(q) - fe * gfac;
       const double d[2] = {t[CT_E0] - u0, t[CT_E1] - u1};
       for (int x = 0; x < 2; x++)
         for (int y = 0; y < 2; y++)
@@ -152,7 +155,7 @@ __device__ __forceinline__ void comp_collide(double (&fl)[Q], double (&gl)[Q], c
   }
   kn = kn / Q;
   const double cs6 = 6.0 * cs2 * cs2 * cs2;
-#pragma unroll
+#pragma unroll 1
   for (int q = 0; q < Q; q++) {
     const double* t = tab + q * CTAB;
     const double fe = qn.feq(t, cs2);
@@ -166,8 +169,8 @@ __device__ __forceinline__ void comp_collide(double (&fl)[Q], double (&gl)[Q], c
             fstar += t[CT_W] * (Qt[x][y][z] * (e[x] * e[y] * e[z] - 3 * cs2 * e[z] * (x == y ? 1.0 : 0.0))) / cs6;
       gstar = t[CT_W] * (Fg[0] * e[0]) / temp + t[CT_W] * (Fg[1] * e[1]) / temp;
     }
-    fl[q] -= om_v * (fl[q] - fe) - om_diff * fstar;
-    gl[q] -= om_v * (gl[q] - ge) - om_diff * gstar;
+    const double fq = fl(q), gq = gl(q);
+    out(q, fq - (om_v * (fq - fe) - om_diff * fstar), gq - (om_v * (gq - ge) - om_diff * gstar));
   }
   if (write_macro) {
     a.rho[node] = (T)rho;
@@ -192,6 +195,18 @@ __device__ __forceinline__ void lag_all(const double (&x)[MAXP + 1], const doubl
 }
 
 // fused SL stream (f and g, general backend records) + compressible collision
+// Register-array form: fl, gl relaxed in place.
+template <int Q, typename T>
+__device__ __forceinline__ void comp_collide(double (&fl)[Q], double (&gl)[Q], const CompArgs<T>& a,
+                                             const double* __restrict__ tab, int64_t node, bool write_macro) {
+  comp_collide_acc<Q, T>([&](int q) { return fl[q]; }, [&](int q) { return gl[q]; },
+                         [&](int q, double fn, double gn) {
+                           fl[q] = fn;
+                           gl[q] = gn;
+                         },
+                         a, tab, node, write_macro);
+}
+
 // Streamed f and g of direction al at DoF i (al uniform across the wave), with the
 // FP32 shift added back. One set of FP64 weights serves f and g.
 template <int Q, int P, typename T>
@@ -294,18 +309,12 @@ __global__ __launch_bounds__(CW * 64) void k_sl_comp_wide(CompArgs<T> a, const d
   }
   __syncthreads();
   if (w != 0 || !live) return;
-  double fl[Q], gl[Q];
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    fl[q] = sf[q][lane];
-    gl[q] = sg[q][lane];
-  }
-  comp_collide<Q, T>(fl, gl, a, tab, i, write_macro != 0);
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    a.fout[(int64_t)q * a.stride + i] = (T)(fl[q] - tab[q * CTAB + CT_W] * a.rho0);
-    a.gout[(int64_t)q * a.stride + i] = (T)(gl[q] - tab[q * CTAB + CT_W] * a.gshift);
-  }
+  comp_collide_acc<Q, T>([&](int q) { return sf[q][lane]; }, [&](int q) { return sg[q][lane]; },
+                         [&](int q, double fn, double gn) {
+                           a.fout[(int64_t)q * a.stride + i] = (T)(fn - tab[q * CTAB + CT_W] * a.rho0);
+                           a.gout[(int64_t)q * a.stride + i] = (T)(gn - tab[q * CTAB + CT_W] * a.gshift);
+                         },
+                         a, tab, i, write_macro != 0);
 }
 
 // collision only, in place
