@@ -48,7 +48,10 @@ __global__ __launch_bounds__(256) void k_sums(SumArgs s, const T* __restrict__ r
     int mi[3] = {l % np, (l / np) % np, s.m.D == 3 ? l / (np * np) : 0};
     double xi[3] = {s.gll[mi[0]], s.gll[mi[1]], s.m.D == 3 ? s.gll[mi[2]] : 0.0};
     double J[3][3];
-    cell_jac(s.m, c, xi, J);
+    if (s.m.D == 3)
+      cell_jac<3>(s.m, c, xi, J);
+    else
+      cell_jac<2>(s.m, c, xi, J);
     double det = s.m.D == 2 ? J[0][0] * J[1][1] - J[0][1] * J[1][0]
                             : J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                                   J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +

@@ -97,7 +97,7 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
     const double xi[3] = {s.gll[mi[0]], s.gll[mi[1]], (D == 3) ? s.gll[mi[2]] : 0.0};
     double J[3][3];
     if (active) {
-      cell_jac(s.m, c, xi, J);
+      cell_jac<D>(s.m, c, xi, J);
     } else {
       for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) J[a][b] = (a == b) ? 1.0 : 0.0;

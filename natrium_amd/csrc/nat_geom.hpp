@@ -5,31 +5,40 @@
 namespace nat {
 
 // Jacobian d x / d xi of cell c at unit coordinates xi (MappingQ1, or
-// MappingCartesian from the first and last vertex when m.cartesian).
+// MappingCartesian from the first and last vertex when m.cartesian). D is the mesh
+// dimension (m.D) as a constant, so J stays in registers.
+template <int D>
 __device__ __forceinline__ void cell_jac(const MeshDev& m, const int c[3], const double xi[3], double J[3][3]) {
+#pragma unroll
   for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) J[a][b] = (a == b && a >= m.D) ? 1.0 : 0.0;
-  const int nv = m.D == 3 ? 8 : 4;
+#pragma unroll
+    for (int b = 0; b < 3; b++) J[a][b] = (a == b && a >= D) ? 1.0 : 0.0;
+  constexpr int nv = D == 3 ? 8 : 4;
   auto V = [&](int v) {
     return m.vert + 3 * ((size_t)(c[0] + (v & 1)) +
                          (size_t)(m.N[0] + 1) * ((size_t)(c[1] + ((v >> 1) & 1)) +
-                                                 (size_t)(m.N[1] + 1) * (m.D == 3 ? c[2] + ((v >> 2) & 1) : 0)));
+                                                 (size_t)(m.N[1] + 1) * (D == 3 ? c[2] + ((v >> 2) & 1) : 0)));
   };
   if (m.cartesian) {
     const double* a0 = V(0);
     const double* b0 = V(nv - 1);
-    for (int d = 0; d < m.D; d++) J[d][d] = b0[d] - a0[d];
+#pragma unroll
+    for (int d = 0; d < D; d++) J[d][d] = b0[d] - a0[d];
     return;
   }
+#pragma unroll
   for (int v = 0; v < nv; v++) {
     const double* X = V(v);
-    for (int b = 0; b < m.D; b++) {
+#pragma unroll
+    for (int b = 0; b < D; b++) {
       double dN = 1;
-      for (int a = 0; a < m.D; a++) {
+#pragma unroll
+      for (int a = 0; a < D; a++) {
         const int bit = (v >> a) & 1;
         dN *= (a == b) ? (bit ? 1.0 : -1.0) : (bit ? xi[a] : 1 - xi[a]);
       }
-      for (int d = 0; d < m.D; d++) J[d][b] += dN * X[d];
+#pragma unroll
+      for (int d = 0; d < D; d++) J[d][b] += dN * X[d];
     }
   }
 }
