@@ -18,11 +18,12 @@
 // FP32 storage keeps f~ = f - w rho0 and g~ = g - w rho0 (2 C_v - D), the
 // equilibria at rest with T = 1; the collision un-shifts in FP64.
 #include "nat_collide.hpp"
+#include "nat_fastdiv.hpp"
 
 namespace nat {
 
 constexpr int CTAB = 16;  // doubles per direction in the constant table
-enum { CT_E0 = 0, CT_E1 = 1, CT_W = 2, CT_EE = 3, CT_H3 = 4, CT_H4 = 8 };
+enum { CT_E0 = 0, CT_E1 = 1, CT_W = 2, CT_EE = 3, CT_H3 = 4, CT_H4 = 8, CT_IW = 13 };
 
 template <typename T>
 struct CompArgs {
@@ -34,6 +35,7 @@ struct CompArgs {
   int64_t n;
   int nfe[3];
   int N[3];
+  FastDivU divN0;  // division by N[0]
   const int* cell;    // [(alpha-1) * n + i] departure cell (-1: identity row)
   const double* xi;   // unit coordinates, 2 per record
   const int* wall_idx;        // walls: wall entry per DoF or -1 (null: no walls)
@@ -43,6 +45,9 @@ struct CompArgs {
   double gll[MAXP + 1];
   double inv_den[MAXP + 1];
   double cs2;         // unscaled c_s^2
+  double ics2;        // 1 / cs2
+  double k3, k4;      // 1 / (6 cs2^3), 1 / (24 cs2^4)
+  double cv;          // C_v = 1 / (gamma - 1)
   double tau;
   double gamma;
   double prandtl;
@@ -59,55 +64,83 @@ struct CompArgs {
   int* status;
 };
 
-// Quartic equilibrium of direction q for the node scalars (2D).
+// Quartic equilibrium of direction q for the node scalars (2D). The node factors
+// (powers of 1/cs2 and rho) are folded into the Hermite coefficients once per node,
+// so a direction costs multiply-adds only (no divisions).
 struct QuarticNode {
-  double rho, u0, u1, uu_term, Tm1, T1;
-  double a_xxx, a_xxy, a_xyy, a_yyy, a_xxxx, a_yyyy, a_xxxy, a_xyyy, a_xxyy;
-  __device__ void init(double r, double x, double y, double T, double cs2) {
+  double rho, us0, us1, uu_term, k2a, k2b;
+  double b_xxx, b_xxy, b_xyy, b_yyy, b_xxxx, b_yyyy, b_xxxy, b_xyyy, b_xxyy;
+  template <typename T>
+  __device__ void init(double r, double x, double y, double T_, const CompArgs<T>& a) {
+    const double cs2 = a.cs2, ics2 = a.ics2;
     rho = r;
-    u0 = x;
-    u1 = y;
-    uu_term = -(x * x) / (2.0 * cs2) - (y * y) / (2.0 * cs2);
-    Tm1 = T - 1;
-    T1 = cs2 * (T - 1);
-    a_xxx = x * x * x + T1 * (x + x + x);
-    a_xxy = x * x * y + T1 * y;
-    a_xyy = x * y * y + T1 * x;
-    a_yyy = y * y * y + T1 * (y + y + y);
-    a_xxxx = x * x * x * x + T1 * x * x * 6.0 + T1 * T1 * 3.0;
-    a_yyyy = y * y * y * y + T1 * y * y * 6.0 + T1 * T1 * 3.0;
-    a_xxxy = x * x * x * y + T1 * (x * y * 3.0);
-    a_xyyy = x * y * y * y + T1 * (x * y * 3.0);
-    a_xxyy = x * x * y * y + T1 * (x * x + y * y) + T1 * T1;
+    us0 = x * ics2;
+    us1 = y * ics2;
+    uu_term = -(x * x) * (0.5 * ics2) - (y * y) * (0.5 * ics2);
+    const double Tm1 = T_ - 1;
+    // rho w (Tm1 |e|^2 - 2 cs2 Tm1) / (2 cs2)
+    k2a = r * Tm1 * (0.5 * ics2);
+    k2b = r * Tm1;
+    const double T1 = cs2 * (T_ - 1);
+    const double r3 = r * a.k3, r4 = r * a.k4;
+    b_xxx = r3 * (x * x * x + T1 * (x + x + x));
+    b_xxy = 3.0 * r3 * (x * x * y + T1 * y);
+    b_xyy = 3.0 * r3 * (x * y * y + T1 * x);
+    b_yyy = r3 * (y * y * y + T1 * (y + y + y));
+    b_xxxx = r4 * (x * x * x * x + T1 * x * x * 6.0 + T1 * T1 * 3.0);
+    b_yyyy = r4 * (y * y * y * y + T1 * y * y * 6.0 + T1 * T1 * 3.0);
+    b_xxxy = 4.0 * r4 * (x * x * x * y + T1 * (x * y * 3.0));
+    b_xyyy = 4.0 * r4 * (x * y * y * y + T1 * (x * y * 3.0));
+    b_xxyy = 6.0 * r4 * (x * x * y * y + T1 * (x * x + y * y) + T1 * T1);
   }
-  __device__ double feq(const double* __restrict__ t, double cs2) const {
-    const double e0 = t[CT_E0], e1 = t[CT_E1], w = t[CT_W];
-    const double ue = (u0 * e0) / cs2 + (u1 * e1) / cs2;
-    double f = w * rho * (1 + ue * (1 + 0.5 * ue) + uu_term);
-    f += rho * w / (2.0 * cs2) * (Tm1 * t[CT_EE] - 2.0 * cs2 * Tm1);
-    f += w * rho / (6. * cs2 * cs2 * cs2) *
-         (a_xxx * t[CT_H3 + 0] + 3 * (a_xxy * t[CT_H3 + 1] + a_xyy * t[CT_H3 + 2]) + a_yyy * t[CT_H3 + 3]);
-    f += w * rho / (24. * cs2 * cs2 * cs2 * cs2) *
-         (t[CT_H4 + 0] * a_xxxx + t[CT_H4 + 1] * a_yyyy + 6.0 * t[CT_H4 + 2] * a_xxyy + 4.0 * t[CT_H4 + 3] * a_xyyy +
-          4.0 * t[CT_H4 + 4] * a_xxxy);
-    return f;
+  __device__ double feq(const double* __restrict__ t) const {
+    const double ue = us0 * t[CT_E0] + us1 * t[CT_E1];
+    double f = rho * (1 + ue * (1 + 0.5 * ue) + uu_term);
+    f += k2a * t[CT_EE] - k2b;
+    f += b_xxx * t[CT_H3 + 0] + b_xxy * t[CT_H3 + 1] + b_xyy * t[CT_H3 + 2] + b_yyy * t[CT_H3 + 3];
+    f += b_xxxx * t[CT_H4 + 0] + b_yyyy * t[CT_H4 + 1] + b_xxyy * t[CT_H4 + 2] + b_xyyy * t[CT_H4 + 3] +
+         b_xxxy * t[CT_H4 + 4];
+    return t[CT_W] * f;
   }
 };
 
-// Compressible collision of one node. fl(q), gl(q) give the unshifted FP64
-// populations; out(q, f_new, g_new) receives the relaxed ones. The accessors let the
-// values live in registers or in LDS (the fused step keeps them in LDS, which leaves
-// the registers to the gathers of the block's other waves).
-template <int Q, typename T, typename FA, typename GA, typename OUT>
-__device__ __forceinline__ void comp_collide_acc(FA fl, GA gl, OUT out, const CompArgs<T>& a,
-                                                 const double* __restrict__ tab, int64_t node, bool write_macro) {
+// Node state of the compressible collision: moments, temperature, equilibrium
+// coefficients and relaxation rates. fl(q), gl(q) give the unshifted FP64 populations
+// (the accessors let the values live in registers or in LDS).
+struct CompNode {
+  double rho, u0, u1, temp, itemp, gfac, om_v, om_diff;
+  QuarticNode qn;
+};
+
+// Node state from complete sums: rho, u, and tf = sum |e - u|^2 f, tg = sum g.
+template <typename T>
+__device__ __forceinline__ CompNode comp_node_sums(double rho, double u0, double u1, double tf, double tg,
+                                                   const CompArgs<T>& a) {
   constexpr int D = 2;
-  const double cs2 = a.cs2;
-  const double C_v = 1. / (a.gamma - 1.0);
+  const double C_v = a.cv;
+  CompNode nd;
+  const double temp = (tf * a.ics2 + tg) * 0.5 / (rho * C_v);
+  nd.rho = rho;
+  nd.u0 = u0;
+  nd.u1 = u1;
+  nd.temp = temp;
+  nd.itemp = 1.0 / temp;
+  nd.qn.init(rho, u0, u1, temp, a);
+  nd.gfac = temp * (2.0 * C_v - D);
+  double S = 1.0;
+  if (a.sutherland) S = pow(temp / 0.85, 0.7);
+  const double visc_tau = (a.tau - 0.5) * S / (temp * rho) + 0.5;
+  const double prandtl_tau = (visc_tau - 0.5) / (a.prandtl_set ? a.prandtl : 1.0) + 0.5;
+  nd.om_v = 1. / visc_tau;
+  nd.om_diff = nd.om_v - 1. / prandtl_tau;
+  return nd;
+}
+
+template <int Q, typename T, typename FA, typename GA>
+__device__ __forceinline__ CompNode comp_node(FA fl, GA gl, const CompArgs<T>& a, const double* __restrict__ tab) {
   double rho = 0;
 #pragma unroll 1
   for (int q = 0; q < Q; q++) rho += fl(q);
-  if (!(rho >= 1e-10)) atomicOr(a.status, 1);
   double j0 = 0, j1 = 0;
 #pragma unroll 1
   for (int q = 0; q < Q; q++) {
@@ -115,70 +148,94 @@ __device__ __forceinline__ void comp_collide_acc(FA fl, GA gl, OUT out, const Co
     j1 += tab[q * CTAB + CT_E1] * fl(q);
   }
   const double u0 = j0 * (1.0 / rho), u1 = j1 * (1.0 / rho);
-  double tsum = 0;
+  double tf = 0, tg = 0;
 #pragma unroll 1
   for (int q = 0; q < Q; q++) {
     const double d0 = tab[q * CTAB + CT_E0] - u0, d1 = tab[q * CTAB + CT_E1] - u1;
-    tsum += (d0 * d0 + d1 * d1) * fl(q) / cs2 + gl(q);
+    tf += (d0 * d0 + d1 * d1) * fl(q);
+    tg += gl(q);
   }
-  const double temp = tsum * 0.5 / (rho * C_v);
-  QuarticNode qn;
-  qn.init(rho, u0, u1, temp, cs2);
-  const double gfac = temp * (2.0 * C_v - D);
+  return comp_node_sums<T>(rho, u0, u1, tf, tg, a);
+}
 
-  double S = 1.0;
-  if (a.sutherland) S = pow(temp / 0.85, 0.7);
-  const double visc_tau = (a.tau - 0.5) * S / (temp * rho) + 0.5;
-  const double prandtl_tau = (visc_tau - 0.5) / (a.prandtl_set ? a.prandtl : 1.0) + 0.5;
-  const double om_v = 1. / visc_tau;
-  const double om_diff = om_v - 1. / prandtl_tau;
-
-  // Prandtl moments of the non-equilibrium parts
+// Sums over directions of the non-equilibrium parts: the Knudsen estimate and, with
+// a Prandtl number (PR), the third moments Q of f_neq and first moments F of g_neq
+// about u.
+struct CompNeq {
+  double kn = 0;
   double Qt[2][2][2] = {{{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}};
   double Fg[2] = {0, 0};
-  double kn = 0;
+};
+
+// Adds direction t's terms; returns its f_eq.
+template <bool PR, typename T>
+__device__ __forceinline__ double comp_neq_add(CompNeq& s, const CompNode& nd, const CompArgs<T>& a,
+                                               const double* __restrict__ t, double fq, double gq) {
+  const double fe = nd.qn.feq(t);
+  const double fneq = fq - fe;
+  s.kn += fabs(fneq) * t[CT_IW];
+  if (PR && a.prandtl_set) {
+    const double gneq = gq - fe * nd.gfac;
+    const double d[2] = {t[CT_E0] - nd.u0, t[CT_E1] - nd.u1};
+    for (int x = 0; x < 2; x++)
+      for (int y = 0; y < 2; y++)
+        for (int z = 0; z < 2; z++) s.Qt[x][y][z] += (d[x] * d[y] * d[z]) * fneq;
+    s.Fg[0] += d[0] * gneq;
+    s.Fg[1] += d[1] * gneq;
+  }
+  return fe;
+}
+
+// Relaxed f and g of direction t, given its f_eq and the complete sums s.
+template <bool PR, typename T>
+__device__ __forceinline__ void comp_relax(const CompNode& nd, const CompNeq& s, const CompArgs<T>& a,
+                                           const double* __restrict__ t, double fe, double fq, double gq, double& fn,
+                                           double& gn) {
+  const double cs2 = a.cs2;
+  const double ge = fe * nd.gfac;
+  double fstar = 0, gstar = 0;
+  if (PR && a.prandtl_set) {
+    // w Q:(e e e - 3 cs2 e I) / (6 cs2^3) with 1 / (6 cs2^3) = k3
+    const double e[2] = {t[CT_E0], t[CT_E1]};
+    double acc = 0;
+    for (int x = 0; x < 2; x++)
+      for (int y = 0; y < 2; y++)
+        for (int z = 0; z < 2; z++) acc += s.Qt[x][y][z] * (e[x] * e[y] * e[z] - 3 * cs2 * e[z] * (x == y ? 1.0 : 0.0));
+    fstar = t[CT_W] * acc * a.k3;
+    gstar = t[CT_W] * (s.Fg[0] * e[0] + s.Fg[1] * e[1]) * nd.itemp;
+  }
+  fn = fq - (nd.om_v * (fq - fe) - nd.om_diff * fstar);
+  gn = gq - (nd.om_v * (gq - ge) - nd.om_diff * gstar);
+}
+
+template <typename T>
+__device__ __forceinline__ void comp_write_macro(const CompArgs<T>& a, const CompNode& nd, double kn_sum, int Q,
+                                                 int64_t node) {
+  a.rho[node] = (T)nd.rho;
+  a.u[node] = (T)(nd.u0 * a.scaling);
+  a.u[a.mstride + node] = (T)(nd.u1 * a.scaling);
+  a.temp[node] = nd.temp;
+  a.mask[node] = kn_sum / Q;
+}
+
+// Compressible collision of one node; out(q, f_new, g_new) receives the relaxed
+// populations.
+template <int Q, typename T, typename FA, typename GA, typename OUT>
+__device__ __forceinline__ void comp_collide_acc(FA fl, GA gl, OUT out, const CompArgs<T>& a,
+                                                 const double* __restrict__ tab, int64_t node, bool write_macro) {
+  const CompNode nd = comp_node<Q, T>(fl, gl, a, tab);
+  if (!(nd.rho >= 1e-10)) atomicOr(a.status, 1);
+  CompNeq s;
+#pragma unroll 1
+  for (int q = 0; q < Q; q++) comp_neq_add<true, T>(s, nd, a, tab + q * CTAB, fl(q), gl(q));
 #pragma unroll 1
   for (int q = 0; q < Q; q++) {
     const double* t = tab + q * CTAB;
-    const double fe = qn.feq(t, cs2);
-    const double fneq = fl(q) - fe;
-    kn += fabs(fneq) / t[CT_W];
-    if (a.prandtl_set) {
-      const double gneq = gl(q) - fe * gfac;
-      const double d[2] = {t[CT_E0] - u0, t[CT_E1] - u1};
-      for (int x = 0; x < 2; x++)
-        for (int y = 0; y < 2; y++)
-          for (int z = 0; z < 2; z++) Qt[x][y][z] += (d[x] * d[y] * d[z]) * fneq;
-      Fg[0] += d[0] * gneq;
-      Fg[1] += d[1] * gneq;
-    }
+    double fn, gn;
+    comp_relax<true, T>(nd, s, a, t, nd.qn.feq(t), fl(q), gl(q), fn, gn);
+    out(q, fn, gn);
   }
-  kn = kn / Q;
-  const double cs6 = 6.0 * cs2 * cs2 * cs2;
-#pragma unroll 1
-  for (int q = 0; q < Q; q++) {
-    const double* t = tab + q * CTAB;
-    const double fe = qn.feq(t, cs2);
-    const double ge = fe * gfac;
-    double fstar = 0, gstar = 0;
-    if (a.prandtl_set) {
-      const double e[2] = {t[CT_E0], t[CT_E1]};
-      for (int x = 0; x < 2; x++)
-        for (int y = 0; y < 2; y++)
-          for (int z = 0; z < 2; z++)
-            fstar += t[CT_W] * (Qt[x][y][z] * (e[x] * e[y] * e[z] - 3 * cs2 * e[z] * (x == y ? 1.0 : 0.0))) / cs6;
-      gstar = t[CT_W] * (Fg[0] * e[0]) / temp + t[CT_W] * (Fg[1] * e[1]) / temp;
-    }
-    const double fq = fl(q), gq = gl(q);
-    out(q, fq - (om_v * (fq - fe) - om_diff * fstar), gq - (om_v * (gq - ge) - om_diff * gstar));
-  }
-  if (write_macro) {
-    a.rho[node] = (T)rho;
-    a.u[node] = (T)(u0 * a.scaling);
-    a.u[a.mstride + node] = (T)(u1 * a.scaling);
-    a.temp[node] = temp;
-    a.mask[node] = kn;
-  }
+  if (write_macro) comp_write_macro<T>(a, nd, s.kn, Q, node);
 }
 
 template <int P>
@@ -227,7 +284,7 @@ __device__ __forceinline__ void comp_stream_dir(const CompArgs<T>& a, const doub
     double lx[P + 1], ly[P + 1];
     lag_all<P>(a.gll, a.inv_den, a.xi[2 * r + 0], lx);
     lag_all<P>(a.gll, a.inv_den, a.xi[2 * r + 1], ly);
-    const int cx = c % a.N[0], cy = c / a.N[0];
+    const int cy = (int)a.divN0.div((uint32_t)c), cx = c - cy * a.N[0];
     const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P);
     af = 0;
     ag = 0;
@@ -276,15 +333,22 @@ __device__ __forceinline__ void comp_stream_dir(const CompArgs<T>& a, const doub
   go = ag + tab[al * CTAB + CT_W] * a.gshift;
 }
 
-// Fused SL stream of f and g + compressible collision. The gathers are spread over the
-// waves of a block: 64 DoFs per block, CW waves each streaming every CW-th direction of
+// Fused SL stream of f and g + compressible collision. The work of a block is spread
+// over its waves: 64 DoFs per block, CW waves each streaming every CW-th direction of
 // those DoFs (one thread per DoF would give a 66k-DoF grid only four waves per CU to hide
-// its gather latency). The streamed values meet in LDS and the first wave collides.
+// its gather latency). The streamed values meet in LDS. The collision is spread the
+// same way: each wave sums the moments over its own directions, the partial sums meet in
+// LDS and are added in wave order (so a run is reproducible), every wave forms the node
+// state from them and relaxes its own directions. Without a Prandtl number the
+// equilibrium of a direction serves both its Knudsen term and its relaxation; with one
+// (PR) the third moments must be complete first, through one more LDS reduction.
 constexpr int CW = 8;
-template <int Q, int P, typename T>
+template <int Q, int P, typename T, bool PR>
 __global__ __launch_bounds__(CW * 64) void k_sl_comp_wide(CompArgs<T> a, const double* __restrict__ tab,
                                                           int write_macro) {
+  constexpr int NV = PR ? 11 : 5;  // kn, Q (8), F (2); moments: rho, j (3) + tf, tg (2)
   __shared__ double sf[Q][64], sg[Q][64];
+  __shared__ double red[CW][NV][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t i = (int64_t)blockIdx.x * 64 + lane;
@@ -308,13 +372,100 @@ __global__ __launch_bounds__(CW * 64) void k_sl_comp_wide(CompArgs<T> a, const d
     }
   }
   __syncthreads();
-  if (w != 0 || !live) return;
-  comp_collide_acc<Q, T>([&](int q) { return sf[q][lane]; }, [&](int q) { return sg[q][lane]; },
-                         [&](int q, double fn, double gn) {
-                           a.fout[(int64_t)q * a.stride + i] = (T)(fn - tab[q * CTAB + CT_W] * a.rho0);
-                           a.gout[(int64_t)q * a.stride + i] = (T)(gn - tab[q * CTAB + CT_W] * a.gshift);
-                         },
-                         a, tab, i, write_macro != 0);
+  const auto fl = [&](int q) { return sf[q][lane]; };
+  const auto gl = [&](int q) { return sg[q][lane]; };
+  // moments: each wave sums its own directions; partial sums meet in LDS, added in wave order
+  double m0 = 0, m1 = 0, m2 = 0;
+  if (live)
+    for (int q = w; q < Q; q += CW) {
+      const double f = fl(q);
+      m0 += f;
+      m1 += tab[q * CTAB + CT_E0] * f;
+      m2 += tab[q * CTAB + CT_E1] * f;
+    }
+  red[w][0][lane] = m0;
+  red[w][1][lane] = m1;
+  red[w][2][lane] = m2;
+  __syncthreads();
+  double rho = red[0][0][lane], j0 = red[0][1][lane], j1 = red[0][2][lane];
+  for (int v = 1; v < CW; v++) {
+    rho += red[v][0][lane];
+    j0 += red[v][1][lane];
+    j1 += red[v][2][lane];
+  }
+  const double u0 = j0 * (1.0 / rho), u1 = j1 * (1.0 / rho);
+  double tf = 0, tg = 0;
+  if (live)
+    for (int q = w; q < Q; q += CW) {
+      const double d0 = tab[q * CTAB + CT_E0] - u0, d1 = tab[q * CTAB + CT_E1] - u1;
+      tf += (d0 * d0 + d1 * d1) * fl(q);
+      tg += gl(q);
+    }
+  red[w][3][lane] = tf;
+  red[w][4][lane] = tg;
+  __syncthreads();
+  tf = red[0][3][lane];
+  tg = red[0][4][lane];
+  for (int v = 1; v < CW; v++) {
+    tf += red[v][3][lane];
+    tg += red[v][4][lane];
+  }
+  CompNode nd = {};
+  if (live) nd = comp_node_sums<T>(rho, u0, u1, tf, tg, a);
+  if (w == 0 && live && !(nd.rho >= 1e-10)) atomicOr(a.status, 1);
+  auto store = [&](int q, double fn, double gn) {
+    a.fout[(int64_t)q * a.stride + i] = (T)(fn - tab[q * CTAB + CT_W] * a.rho0);
+    a.gout[(int64_t)q * a.stride + i] = (T)(gn - tab[q * CTAB + CT_W] * a.gshift);
+  };
+  CompNeq s;
+  if constexpr (PR) {
+    if (live)
+      for (int q = w; q < Q; q += CW) comp_neq_add<true, T>(s, nd, a, tab + q * CTAB, fl(q), gl(q));
+    __syncthreads();  // every wave has read the moment sums
+    red[w][0][lane] = s.kn;
+#pragma unroll
+    for (int k = 0; k < 8; k++) red[w][1 + k][lane] = s.Qt[k >> 2][(k >> 1) & 1][k & 1];
+    red[w][9][lane] = s.Fg[0];
+    red[w][10][lane] = s.Fg[1];
+    __syncthreads();
+    CompNeq t;
+    t.kn = red[0][0][lane];
+#pragma unroll
+    for (int k = 0; k < 8; k++) t.Qt[k >> 2][(k >> 1) & 1][k & 1] = red[0][1 + k][lane];
+    t.Fg[0] = red[0][9][lane];
+    t.Fg[1] = red[0][10][lane];
+    for (int v = 1; v < CW; v++) {
+      t.kn += red[v][0][lane];
+#pragma unroll
+      for (int k = 0; k < 8; k++) t.Qt[k >> 2][(k >> 1) & 1][k & 1] += red[v][1 + k][lane];
+      t.Fg[0] += red[v][9][lane];
+      t.Fg[1] += red[v][10][lane];
+    }
+    if (live)
+      for (int q = w; q < Q; q += CW) {
+        const double* tq = tab + q * CTAB;
+        double fn, gn;
+        comp_relax<true, T>(nd, t, a, tq, nd.qn.feq(tq), fl(q), gl(q), fn, gn);
+        store(q, fn, gn);
+      }
+    s.kn = t.kn;
+  } else {
+    if (live)
+      for (int q = w; q < Q; q += CW) {
+        const double* tq = tab + q * CTAB;
+        const double fe = comp_neq_add<false, T>(s, nd, a, tq, fl(q), gl(q));
+        double fn, gn;
+        comp_relax<false, T>(nd, s, a, tq, fe, fl(q), gl(q), fn, gn);
+        store(q, fn, gn);
+      }
+    if (write_macro) {  // uniform over the block
+      red[w][0][lane] = s.kn;
+      __syncthreads();
+      s.kn = red[0][0][lane];
+      for (int v = 1; v < CW; v++) s.kn += red[v][0][lane];
+    }
+  }
+  if (write_macro && w == 0 && live) comp_write_macro<T>(a, nd, s.kn, Q, i);
 }
 
 // collision only, in place
@@ -348,6 +499,7 @@ static void fill_comp(nat_ctx* ctx, CompArgs<T>& a, int src, int dst) {
     a.nfe[d] = ctx->nfe[d];
     a.N[d] = d < ctx->D ? ctx->desc.ncells[d] : 1;
   }
+  a.divN0.set((uint32_t)a.N[0]);
   a.cell = ctx->dep_cell;
   a.xi = ctx->dep_xi64;
   a.wall_idx = ctx->has_walls ? ctx->wall_idx : nullptr;
@@ -370,6 +522,10 @@ static void fill_comp(nat_ctx* ctx, CompArgs<T>& a, int src, int dst) {
   }
   const double s = ctx->st.s;
   a.cs2 = ctx->st.cs2 / (s * s);
+  a.ics2 = 1.0 / a.cs2;
+  a.k3 = 1.0 / (6.0 * a.cs2 * a.cs2 * a.cs2);
+  a.k4 = 1.0 / (24.0 * a.cs2 * a.cs2 * a.cs2 * a.cs2);
+  a.cv = 1.0 / (ctx->desc.gamma - 1.0);
   a.tau = ctx->tau;
   a.gamma = ctx->desc.gamma;
   a.prandtl = ctx->desc.prandtl;
@@ -396,15 +552,24 @@ static int step_comp(nat_ctx* ctx, int write_macro) {
   const unsigned grid = (unsigned)((a.n + 63) / 64);
   const double* tab = ctx->comp_tab;
   const dim3 blk(CW * 64);
+#define NAT_CW(PP)                                                                                              \
+  case PP:                                                                                                     \
+    if (a.prandtl_set)                                                                                         \
+      hipLaunchKernelGGL((k_sl_comp_wide<25, PP, T, true>), dim3(grid), blk, 0, ctx->stream, a, tab, write_macro); \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_sl_comp_wide<25, PP, T, false>), dim3(grid), blk, 0, ctx->stream, a, tab,            \
+                         write_macro);                                                                         \
+    break;
   switch (ctx->p) {
-    case 1: hipLaunchKernelGGL((k_sl_comp_wide<25, 1, T>), dim3(grid), blk, 0, ctx->stream, a, tab, write_macro); break;
-    case 2: hipLaunchKernelGGL((k_sl_comp_wide<25, 2, T>), dim3(grid), blk, 0, ctx->stream, a, tab, write_macro); break;
-    case 3: hipLaunchKernelGGL((k_sl_comp_wide<25, 3, T>), dim3(grid), blk, 0, ctx->stream, a, tab, write_macro); break;
-    case 4: hipLaunchKernelGGL((k_sl_comp_wide<25, 4, T>), dim3(grid), blk, 0, ctx->stream, a, tab, write_macro); break;
+    NAT_CW(1)
+    NAT_CW(2)
+    NAT_CW(3)
+    NAT_CW(4)
     default:
       ctx->err = "compressible SL kernel instantiated for p <= 4";
       return NAT_ERR_UNSUPPORTED;
   }
+#undef NAT_CW
   NAT_HIP(hipGetLastError());
   ctx->cur = dst;
   return 0;
@@ -460,6 +625,7 @@ int nat_comp_setup(nat_ctx* ctx) {
     r[CT_H4 + 2] = H4(0, 0, 1, 1);
     r[CT_H4 + 3] = H4(0, 1, 1, 1);
     r[CT_H4 + 4] = H4(0, 0, 0, 1);
+    r[CT_IW] = 1.0 / ctx->st.w[q];
   }
   if (!ctx->comp_tab) NAT_HIP(hipMalloc(&ctx->comp_tab, t.size() * sizeof(double)));
   NAT_HIP(hipMemcpy(ctx->comp_tab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));

@@ -11,6 +11,7 @@
 // packed in one int32 and the offsets from that node in 2^-30 fixed point (4 + 4D
 // bytes), instead of (p+1)^D stored weights plus column indices per row.
 #include "nat_collide.hpp"
+#include "nat_fastdiv.hpp"
 
 namespace nat {
 
@@ -22,6 +23,7 @@ struct GeneralArgs {
   int64_t n;          // FE_Q DoFs
   int nfe[3];         // FE_Q lattice nodes per axis
   int N[3];           // cells per axis
+  FastDivU divN0, divN1;  // division by N[0], N[1]
   const int* cell;    // FP64 storage: [(alpha-1) * n + i] departure cell (-1: identity row)
   const double* xi;   //   unit coordinates, D per record
   const int* code;    // FP32 storage: cell * (p+1)^D + anchor node (-1: identity row)
@@ -138,9 +140,10 @@ __device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, i
     basis_at<P>(a.dgl, a.inv_den, ay, (double)a.off[r * D + 1] * q, ly);
     if constexpr (D == 3) basis_at<P>(a.dgl, a.inv_den, az, (double)a.off[r * D + 2] * q, lz);
   }
-  const int cx = c % a.N[0];
-  const int cy = (c / a.N[0]) % a.N[1];
-  const int cz = (D == 3) ? c / (a.N[0] * a.N[1]) : 0;
+  const uint32_t q0 = a.divN0.div((uint32_t)c), q1 = a.divN1.div(q0);
+  const int cx = c - (int)q0 * a.N[0];
+  const int cy = (int)q0 - (int)q1 * a.N[1];
+  const int cz = (D == 3) ? (int)q1 : 0;
   const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P + (int64_t)n1 * ((int64_t)cz * P));
   double acc = 0;
   // When no product weight can fall under the reference cut-off (product of the
@@ -292,6 +295,8 @@ static void fill_general(nat_ctx* ctx, GeneralArgs<T>& a) {
     a.nfe[d] = ctx->nfe[d];
     a.N[d] = d < ctx->D ? ctx->desc.ncells[d] : 1;
   }
+  a.divN0.set((uint32_t)a.N[0]);
+  a.divN1.set((uint32_t)a.N[1]);
   a.cell = ctx->dep_cell;
   a.xi = ctx->dep_xi64;
   a.code = ctx->dep_code32;
