@@ -233,7 +233,7 @@ void rk5_step(const SLOperator& op, double dt, double* const* f, std::vector<dou
       }
       d *= RK5_B[s];
       f[b][i] += d;
-      d /= RK5_B[s];
+      d *= 1.0 / RK5_B[s];  // operator/= of deal.II / Trilinos vectors: times the reciprocal
       df[t] = d;
     }
   }
