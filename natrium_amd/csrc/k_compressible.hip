@@ -105,8 +105,7 @@ struct QuarticNode {
 };
 
 // Node state of the compressible collision: moments, temperature, equilibrium
-// coefficients and relaxation rates. fl(q), gl(q) give the unshifted FP64 populations
-// (the accessors let the values live in registers or in LDS).
+// coefficients and relaxation rates.
 struct CompNode {
   double rho, u0, u1, temp, itemp, gfac, om_v, om_diff;
   QuarticNode qn;
@@ -134,28 +133,6 @@ __device__ __forceinline__ CompNode comp_node_sums(double rho, double u0, double
   nd.om_v = 1. / visc_tau;
   nd.om_diff = nd.om_v - 1. / prandtl_tau;
   return nd;
-}
-
-template <int Q, typename T, typename FA, typename GA>
-__device__ __forceinline__ CompNode comp_node(FA fl, GA gl, const CompArgs<T>& a, const double* __restrict__ tab) {
-  double rho = 0;
-#pragma unroll 1
-  for (int q = 0; q < Q; q++) rho += fl(q);
-  double j0 = 0, j1 = 0;
-#pragma unroll 1
-  for (int q = 0; q < Q; q++) {
-    j0 += tab[q * CTAB + CT_E0] * fl(q);
-    j1 += tab[q * CTAB + CT_E1] * fl(q);
-  }
-  const double u0 = j0 * (1.0 / rho), u1 = j1 * (1.0 / rho);
-  double tf = 0, tg = 0;
-#pragma unroll 1
-  for (int q = 0; q < Q; q++) {
-    const double d0 = tab[q * CTAB + CT_E0] - u0, d1 = tab[q * CTAB + CT_E1] - u1;
-    tf += (d0 * d0 + d1 * d1) * fl(q);
-    tg += gl(q);
-  }
-  return comp_node_sums<T>(rho, u0, u1, tf, tg, a);
 }
 
 // Sums over directions of the non-equilibrium parts: the Knudsen estimate and, with
@@ -218,26 +195,6 @@ __device__ __forceinline__ void comp_write_macro(const CompArgs<T>& a, const Com
   a.mask[node] = kn_sum / Q;
 }
 
-// Compressible collision of one node; out(q, f_new, g_new) receives the relaxed
-// populations.
-template <int Q, typename T, typename FA, typename GA, typename OUT>
-__device__ __forceinline__ void comp_collide_acc(FA fl, GA gl, OUT out, const CompArgs<T>& a,
-                                                 const double* __restrict__ tab, int64_t node, bool write_macro) {
-  const CompNode nd = comp_node<Q, T>(fl, gl, a, tab);
-  if (!(nd.rho >= 1e-10)) atomicOr(a.status, 1);
-  CompNeq s;
-#pragma unroll 1
-  for (int q = 0; q < Q; q++) comp_neq_add<true, T>(s, nd, a, tab + q * CTAB, fl(q), gl(q));
-#pragma unroll 1
-  for (int q = 0; q < Q; q++) {
-    const double* t = tab + q * CTAB;
-    double fn, gn;
-    comp_relax<true, T>(nd, s, a, t, nd.qn.feq(t), fl(q), gl(q), fn, gn);
-    out(q, fn, gn);
-  }
-  if (write_macro) comp_write_macro<T>(a, nd, s.kn, Q, node);
-}
-
 template <int P>
 __device__ __forceinline__ void lag_all(const double (&x)[MAXP + 1], const double (&inv)[MAXP + 1], double xi,
                                         double (&v)[P + 1]) {
@@ -249,19 +206,6 @@ __device__ __forceinline__ void lag_all(const double (&x)[MAXP + 1], const doubl
       if (m != j) s *= (xi - x[m]);
     v[j] = s;
   }
-}
-
-// fused SL stream (f and g, general backend records) + compressible collision
-// Register-array form: fl, gl relaxed in place.
-template <int Q, typename T>
-__device__ __forceinline__ void comp_collide(double (&fl)[Q], double (&gl)[Q], const CompArgs<T>& a,
-                                             const double* __restrict__ tab, int64_t node, bool write_macro) {
-  comp_collide_acc<Q, T>([&](int q) { return fl[q]; }, [&](int q) { return gl[q]; },
-                         [&](int q, double fn, double gn) {
-                           fl[q] = fn;
-                           gl[q] = gn;
-                         },
-                         a, tab, node, write_macro);
 }
 
 // Streamed f and g of direction al at DoF i (al uniform across the wave), with the
@@ -343,7 +287,11 @@ __device__ __forceinline__ void comp_stream_dir(const CompArgs<T>& a, const doub
 // equilibrium of a direction serves both its Knudsen term and its relaxation; with one
 // (PR) the third moments must be complete first, through one more LDS reduction.
 constexpr int CW = 8;
-template <int Q, int P, typename T, bool PR>
+// ST = false: no streaming, each DoF's own populations are collided in place (the
+// initial collision of CFDSolver::run; fin and fout may be the same arrays, since every
+// value of the block is read into LDS before the barrier and only this block touches
+// its 64 DoFs).
+template <int Q, int P, typename T, bool PR, bool ST = true>
 __global__ __launch_bounds__(CW * 64) void k_sl_comp_wide(CompArgs<T> a, const double* __restrict__ tab,
                                                           int write_macro) {
   constexpr int NV = PR ? 11 : 5;  // kn, Q (8), F (2); moments: rho, j (3) + tf, tg (2)
@@ -355,7 +303,7 @@ __global__ __launch_bounds__(CW * 64) void k_sl_comp_wide(CompArgs<T> a, const d
   const bool live = i < a.n;
   int widx = -1;
   unsigned wmask = 0;
-  if (live && a.wall_idx) {
+  if (ST && live && a.wall_idx) {
     widx = a.wall_idx[i];
     if (widx >= 0) wmask = a.wall_flip[widx];
   }
@@ -366,7 +314,12 @@ __global__ __launch_bounds__(CW * 64) void k_sl_comp_wide(CompArgs<T> a, const d
     }
     for (int al = 1 + w; al < Q; al += CW) {
       double fo, go;
-      comp_stream_dir<Q, P, T>(a, tab, i, al, widx, wmask, fo, go);
+      if constexpr (ST) {
+        comp_stream_dir<Q, P, T>(a, tab, i, al, widx, wmask, fo, go);
+      } else {
+        fo = (double)a.fin[(int64_t)al * a.stride + i] + tab[al * CTAB + CT_W] * a.rho0;
+        go = (double)a.gin[(int64_t)al * a.stride + i] + tab[al * CTAB + CT_W] * a.gshift;
+      }
       sf[al][lane] = fo;
       sg[al][lane] = go;
     }
@@ -468,25 +421,6 @@ __global__ __launch_bounds__(CW * 64) void k_sl_comp_wide(CompArgs<T> a, const d
   if (write_macro && w == 0 && live) comp_write_macro<T>(a, nd, s.kn, Q, i);
 }
 
-// collision only, in place
-template <int Q, typename T>
-__global__ __launch_bounds__(256) void k_collide_comp(CompArgs<T> a, const double* __restrict__ tab) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  double fl[Q], gl[Q];
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    fl[q] = (double)a.fin[(int64_t)q * a.stride + i] + tab[q * CTAB + CT_W] * a.rho0;
-    gl[q] = (double)a.gin[(int64_t)q * a.stride + i] + tab[q * CTAB + CT_W] * a.gshift;
-  }
-  comp_collide<Q, T>(fl, gl, a, tab, i, true);
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    a.fout[(int64_t)q * a.stride + i] = (T)(fl[q] - tab[q * CTAB + CT_W] * a.rho0);
-    a.gout[(int64_t)q * a.stride + i] = (T)(gl[q] - tab[q * CTAB + CT_W] * a.gshift);
-  }
-}
-
 template <typename T>
 static void fill_comp(nat_ctx* ctx, CompArgs<T>& a, int src, int dst) {
   a.fin = (const T*)ctx->f[src];
@@ -579,8 +513,12 @@ template <typename T>
 static int collide_comp(nat_ctx* ctx) {
   CompArgs<T> a;
   fill_comp<T>(ctx, a, ctx->cur, ctx->cur);
-  const unsigned grid = (unsigned)((a.n + 255) / 256);
-  hipLaunchKernelGGL((k_collide_comp<25, T>), dim3(grid), dim3(256), 0, ctx->stream, a, (const double*)ctx->comp_tab);
+  const unsigned grid = (unsigned)((a.n + 63) / 64);
+  const double* tab = ctx->comp_tab;
+  if (a.prandtl_set)
+    hipLaunchKernelGGL((k_sl_comp_wide<25, 1, T, true, false>), dim3(grid), dim3(CW * 64), 0, ctx->stream, a, tab, 1);
+  else
+    hipLaunchKernelGGL((k_sl_comp_wide<25, 1, T, false, false>), dim3(grid), dim3(CW * 64), 0, ctx->stream, a, tab, 1);
   NAT_HIP(hipGetLastError());
   return 0;
 }
