@@ -135,6 +135,23 @@ int nat_stream(nat_ctx* ctx);
  * runs the initial collide() of run() once with nat_collide(). */
 int nat_step(nat_ctx* ctx, int nsteps);
 
+/* The same operations with one entry point per path, for callers that bind a single
+ * path; each returns NAT_ERR_ARG when the context is of another kind.
+ *   nat_sl_assemble: SemiLagrangian::setDeltaT + reassemble (SemiLagrangian.h:175-179,
+ *     SemiLagrangian.cpp:76-90,145-757) of an SL context (incompressible or compressible);
+ *     dt <= 0 keeps the CFL time step. As nat_set_dt.
+ *   nat_sl_bgk_step: nsteps of the run() loop body with the SL stream and the BGK collision
+ *     (CFDSolver.cpp:669-721,807-843,884-888) of an incompressible SL context. As nat_step.
+ *   nat_sedg_rk5_step: nsteps of the loop body whose stream is one low-storage RK5 step of
+ *     the SEDG operator (RungeKutta5LowStorage.cpp:56-105, SEDGMinLee.cpp:321-536). As nat_step.
+ *   nat_compressible_step: nsteps of CompressibleCFDSolver::run's loop body
+ *     (CompressibleCFDSolver.h:181-314,739-1040): stream f and g with the SL operator, then
+ *     the compressible collision. As nat_step. */
+int nat_sl_assemble(nat_ctx* ctx, double dt);
+int nat_sl_bgk_step(nat_ctx* ctx, int nsteps);
+int nat_sedg_rk5_step(nat_ctx* ctx, int nsteps);
+int nat_compressible_step(nat_ctx* ctx, int nsteps);
+
 /* Measurement helper with no counterpart in the reference (its solver prints
  * "million DoF updates per second" from clock(), CFDSolver.cpp:975-982): runs
  * nat_step(ctx, nsteps) between two HIP events recorded on the context's stream

@@ -54,7 +54,8 @@ EXPORTS = [
     "nat_tau", "nat_time", "nat_backend", "nat_node_coords", "nat_set_dt", "nat_set_vertices", "nat_set_stabilizer",
     "nat_init_equilibrium",
     "nat_upload_f", "nat_download_f", "nat_upload_g", "nat_download_g", "nat_collide", "nat_stream",
-    "nat_step", "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sync",
+    "nat_step", "nat_sl_assemble", "nat_sl_bgk_step", "nat_sedg_rk5_step", "nat_compressible_step",
+    "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sync",
     "nat_stats", "nat_write_stats", "nat_write_vtu", "nat_checkpoint_write", "nat_checkpoint_read",
 ]
 
@@ -111,6 +112,11 @@ def lib():
         getattr(L, nm).argtypes = [vp, _dp]
     L.nat_step.restype = ctypes.c_int
     L.nat_step.argtypes = [vp, ctypes.c_int]
+    L.nat_sl_assemble.restype = ctypes.c_int
+    L.nat_sl_assemble.argtypes = [vp, ctypes.c_double]
+    for name in ("nat_sl_bgk_step", "nat_sedg_rk5_step", "nat_compressible_step"):
+        getattr(L, name).restype = ctypes.c_int
+        getattr(L, name).argtypes = [vp, ctypes.c_int]
     L.nat_step_timed.restype = ctypes.c_int
     L.nat_step_timed.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.nat_get_macro.restype = ctypes.c_int

@@ -1011,6 +1011,27 @@ int nat_step(nat_ctx* ctx, int nsteps) {
   return check_status(ctx);
 }
 
+int nat_sl_assemble(nat_ctx* ctx, double dt) {
+  if (ctx->desc.scheme != NAT_SL) return fail(ctx, NAT_ERR_ARG, "nat_sl_assemble: the context uses the SEDG scheme");
+  return nat_set_dt(ctx, dt);
+}
+
+int nat_sl_bgk_step(nat_ctx* ctx, int nsteps) {
+  if (ctx->desc.scheme != NAT_SL || ctx->desc.compressible)
+    return fail(ctx, NAT_ERR_ARG, "nat_sl_bgk_step: the context is not an incompressible SL context");
+  return nat_step(ctx, nsteps);
+}
+
+int nat_sedg_rk5_step(nat_ctx* ctx, int nsteps) {
+  if (ctx->desc.scheme == NAT_SL) return fail(ctx, NAT_ERR_ARG, "nat_sedg_rk5_step: the context uses the SL scheme");
+  return nat_step(ctx, nsteps);
+}
+
+int nat_compressible_step(nat_ctx* ctx, int nsteps) {
+  if (!ctx->desc.compressible) return fail(ctx, NAT_ERR_ARG, "nat_compressible_step: the context is incompressible");
+  return nat_step(ctx, nsteps);
+}
+
 int nat_step_timed(nat_ctx* ctx, int nsteps, double* elapsed_ms) {
   hipEvent_t a, b;
   NAT_HIP(hipEventCreate(&a));
