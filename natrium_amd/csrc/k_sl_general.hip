@@ -15,6 +15,15 @@
 
 namespace nat {
 
+// Workgroup index regrouped so that each XCD (workgroup i is dispatched to XCD i % 8)
+// runs a contiguous range of the grid: the gathers of a block read rows of the blocks
+// one lattice plane away, which then sit in the same L2 (C3: FP32 1.60 -> 1.78 GLUPS,
+// FP64 0.84 -> 0.92).
+__device__ __forceinline__ int64_t xcd_block_index() {
+  const int64_t b = blockIdx.x, per = gridDim.x / 8;
+  return b < per * 8 ? (b % 8) * per + b / 8 : b;
+}
+
 template <typename T>
 struct GeneralArgs {
   const T* fin;
@@ -223,7 +232,7 @@ struct GeneralShape {
 template <int D, int Q, int P, typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GeneralShape<Q, T>::waves))) void k_sl_general(
     GeneralArgs<T> a, CollideArgs<T> ca, int write_macro) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block_index() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   int widx = -1;
   unsigned wmask = 0;
