@@ -472,7 +472,9 @@ static int launch_struct(nat_ctx* ctx, StructArgs<T> a, const CollideArgs<T>& ca
   a.tiles[2] = D == 3 ? (a.n[2] + TZ - 1) / TZ : 1;
   const int nb = a.tiles[0] * a.tiles[1] * a.tiles[2];
   a.xcd_remap = (nb % 8 == 0) ? 1 : 0;
-  constexpr int B = sizeof(T) == 4 ? NAT_STRUCT_B32 : NAT_STRUCT_B64;
+  // FP64 at p = 6: groups of 2 directions (3 hold 3 x 7 FP64 first-pass values per item
+  // slot and spill at the 128-register budget)
+  constexpr int B = sizeof(T) == 4 ? NAT_STRUCT_B32 : (P >= 6 ? 2 : NAT_STRUCT_B64);
   constexpr int WPE = 4;  // 16 waves per CU: one 1024-thread tile, up to 128 VGPRs
   if (SYM_ONLY || ctx->ax_sym)
     hipLaunchKernelGGL((k_sl_struct<D, Q, P, T, TX, TY, TZ, B, WPE, true>), dim3(nb), dim3(TX * TY * TZ), 0,

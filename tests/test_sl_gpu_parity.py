@@ -78,7 +78,8 @@ def test_tgv2d_d2q9_p4_fp64_100_steps_and_sums(backend):
     assert _rel(ug, uo) < 1e-10
 
 
-@pytest.mark.parametrize("stencil,p,N", [("D3Q27", 4, 4), ("D3Q19", 4, 4), ("D3Q27", 2, 4), ("D3Q19", 3, 4)])
+@pytest.mark.parametrize("stencil,p,N", [("D3Q27", 4, 4), ("D3Q19", 4, 4), ("D3Q27", 2, 4), ("D3Q19", 3, 4),
+                                          ("D3Q27", 6, 2), ("D3Q19", 6, 2)])
 @pytest.mark.parametrize("backend", ["structured", "general"])
 def test_tgv3d_fp64_parity(stencil, p, N, backend):
     cs, s = lbm_parameters(1.0, 0.1)
