@@ -286,14 +286,22 @@ __device__ __forceinline__ void comp_stream_dir(const CompArgs<T>& a, const doub
 // state from them and relaxes its own directions. Without a Prandtl number the
 // equilibrium of a direction serves both its Knudsen term and its relaxation; with one
 // (PR) the third moments must be complete first, through one more LDS reduction.
-constexpr int CW = 8;
+// Waves per block: 8, or 4 with a Prandtl number, whose 11-value reduction buffer would
+// otherwise leave room for one block per CU (C4 with Pr = 0.71: 0.97 -> 1.60 GLUPS FP32;
+// 8 waves measured best without it).
+template <bool PR>
+constexpr int comp_waves() {
+  return PR ? 4 : 8;
+}
 // ST = false: no streaming, each DoF's own populations are collided in place (the
 // initial collision of CFDSolver::run; fin and fout may be the same arrays, since every
 // value of the block is read into LDS before the barrier and only this block touches
 // its 64 DoFs).
 template <int Q, int P, typename T, bool PR, bool ST = true>
-__global__ __launch_bounds__(CW * 64) void k_sl_comp_wide(CompArgs<T> a, const double* __restrict__ tab,
-                                                          int write_macro) {
+__global__ __launch_bounds__(comp_waves<PR>() * 64) void k_sl_comp_wide(CompArgs<T> a,
+                                                                        const double* __restrict__ tab,
+                                                                        int write_macro) {
+  constexpr int CW = comp_waves<PR>();
   constexpr int NV = PR ? 11 : 5;  // kn, Q (8), F (2); moments: rho, j (3) + tf, tg (2)
   __shared__ double sf[Q][64], sg[Q][64];
   __shared__ double red[CW][NV][64];
@@ -485,13 +493,13 @@ static int step_comp(nat_ctx* ctx, int write_macro) {
   fill_comp<T>(ctx, a, src, dst);
   const unsigned grid = (unsigned)((a.n + 63) / 64);
   const double* tab = ctx->comp_tab;
-  const dim3 blk(CW * 64);
+  const dim3 blk(comp_waves<true>() * 64), blk0(comp_waves<false>() * 64);
 #define NAT_CW(PP)                                                                                              \
   case PP:                                                                                                     \
     if (a.prandtl_set)                                                                                         \
       hipLaunchKernelGGL((k_sl_comp_wide<25, PP, T, true>), dim3(grid), blk, 0, ctx->stream, a, tab, write_macro); \
     else                                                                                                       \
-      hipLaunchKernelGGL((k_sl_comp_wide<25, PP, T, false>), dim3(grid), blk, 0, ctx->stream, a, tab,            \
+      hipLaunchKernelGGL((k_sl_comp_wide<25, PP, T, false>), dim3(grid), blk0, 0, ctx->stream, a, tab,           \
                          write_macro);                                                                         \
     break;
   switch (ctx->p) {
@@ -516,9 +524,11 @@ static int collide_comp(nat_ctx* ctx) {
   const unsigned grid = (unsigned)((a.n + 63) / 64);
   const double* tab = ctx->comp_tab;
   if (a.prandtl_set)
-    hipLaunchKernelGGL((k_sl_comp_wide<25, 1, T, true, false>), dim3(grid), dim3(CW * 64), 0, ctx->stream, a, tab, 1);
+    hipLaunchKernelGGL((k_sl_comp_wide<25, 1, T, true, false>), dim3(grid), dim3(comp_waves<true>() * 64), 0,
+                       ctx->stream, a, tab, 1);
   else
-    hipLaunchKernelGGL((k_sl_comp_wide<25, 1, T, false, false>), dim3(grid), dim3(CW * 64), 0, ctx->stream, a, tab, 1);
+    hipLaunchKernelGGL((k_sl_comp_wide<25, 1, T, false, false>), dim3(grid), dim3(comp_waves<false>() * 64), 0,
+                       ctx->stream, a, tab, 1);
   NAT_HIP(hipGetLastError());
   return 0;
 }
