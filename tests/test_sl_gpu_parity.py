@@ -155,3 +155,28 @@ def test_density_error_is_reported():
     g.upload_f(f)
     with pytest.raises(nat.DensityZeroError):
         g.collide()
+
+
+@pytest.mark.parametrize("stencil,p,N,tiles", [("D2Q9", 4, 128, (512 // 32) * (512 // 16)),
+                                               ("D3Q19", 2, 32, (64 // 16) * (64 // 8) * (64 // 8))])
+def test_structured_large_tiles_fp64_parity(stencil, p, N, tiles):
+    """Meshes with at least as many 1024-node (2D: 32x16) tiles as the GPU has CUs keep the
+    large tiles of the C2 workload, where smaller meshes take ~256-node tiles: FP64 parity with
+    the oracle per step on 128^2 cells (D2Q9, p=4) and 32^3 cells (D3Q19, p=2)."""
+    assert tiles >= 256
+    D = 2 if stencil == "D2Q9" else 3
+    cs, s = lbm_parameters(1.0, 0.1)
+    L = 2 * math.pi
+    v = box_vertices(D, [N] * D, [0] * D, [L] * D)
+    g, o = _pair(stencil, p, [N] * D, v, True, s, 0.01, backend="structured")
+    x = g.coords()
+    rho, u = tgv2d_fields(x, 0.01, cs, L) if D == 2 else tgv3d_fields(x, cs)
+    g.init_equilibrium(rho, u)
+    o.init(rho, u)
+    g.collide()
+    o.collide()
+    for step in range(1, 4):
+        g.step(1)
+        o.steps(1)
+        e = _rel(g.download_f(), o.get_f())
+        assert e < FP64_TOL, (step, e)
