@@ -66,7 +66,9 @@ template <int D, int P>
 struct SedgShape {
   static constexpr int NP = P + 1;
   static constexpr int NL = (D == 3) ? NP * NP * NP : NP * NP;
-  static constexpr int CPB = (256 / NL) > 0 ? 256 / NL : 1;  // cells per block
+  // cells per block: ~256 threads; a 64-node cell (3D, p = 3) runs alone as one wave, whose
+  // per-direction barrier then waits for no other wave (C3 SEDG +4% FP32, +1% FP64)
+  static constexpr int CPB = (NL == 64) ? 1 : ((256 / NL) > 0 ? 256 / NL : 1);
   static constexpr int NT = CPB * NL;
 };
 
