@@ -183,15 +183,15 @@ def test_structured_large_tiles_fp64_parity(stencil, p, N, tiles):
 
 
 def test_general_regrouped_blocks_fp64_parity():
-    """General kernel on a grid large enough for its XCD block regrouping (62 blocks of 256
-    DoFs: the first 56 regrouped, the last 6 in place): perturbed 8^3 cells, D3Q19, p=3,
+    """General kernel on a grid large enough for its XCD block regrouping (FP64: 123 blocks of
+    128 DoFs, the first 120 regrouped, the last 3 in place): perturbed 8^3 cells, D3Q19, p=3,
     FP64 parity with the oracle per step."""
     cs, s = lbm_parameters(1.0, 0.05)
     L = 2 * math.pi
     N = [8, 8, 8]
     v = perturbed_vertices(3, N, [0] * 3, [L] * 3, amplitude=0.1, seed=42)
     g, o = _pair("D3Q19", 3, N, v, False, s, 0.01, backend="general")
-    assert (g.n + 255) // 256 == 62
+    assert (g.n + 127) // 128 == 123
     x = g.coords()
     rho, u = tgv3d_fields(x, cs)
     g.init_equilibrium(rho, u)
