@@ -394,6 +394,79 @@ __global__ void k_assemble_axis(int D, const int* N, const double* vert_axis /* 
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+namespace nat {
+
+// FE_Q support points of the lattice DoFs (x fastest; SemiLagrangian.cpp:214-229) through
+// the cell map. Each DoF is owned by the lowest cell that contains it: a DoF at index
+// k > 0 of an axis belongs to cell (k - 1) / p. The map is x = sum_v N_v X_v with
+// N_v = prod (xi or 1 - xi), or x = a + xi (b - a) on Cartesian cells. Contraction is off
+// in this kernel: every product and sum is rounded on its own, as a CPU without fused
+// multiply-add forms them, so the points match a CPU evaluation of the same formula bit
+// for bit.
+template <int D>
+__global__ __launch_bounds__(256) void k_support_points(MeshDev m, int p, GllTab g, int nfe0, int nfe1, int64_t n,
+                                                        double* __restrict__ xyz, int* __restrict__ owner) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int idx[3] = {(int)(i % nfe0), (int)((i / nfe0) % nfe1), D == 3 ? (int)(i / ((int64_t)nfe0 * nfe1)) : 0};
+  int cc[3] = {0, 0, 0};
+  double xi[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < D; a++) {
+    cc[a] = idx[a] == 0 ? 0 : (idx[a] - 1) / p;
+    xi[a] = g.x[idx[a] - cc[a] * p];
+  }
+  double x[3] = {0, 0, 0};
+  constexpr int nv = D == 3 ? 8 : 4;
+  if (m.cartesian) {
+    const double* a0 = cell_vertex<D>(m, cc, 0);
+    const double* b0 = cell_vertex<D>(m, cc, nv - 1);
+#pragma unroll
+    for (int d = 0; d < D; d++) x[d] = a0[d] + xi[d] * (b0[d] - a0[d]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < nv; v++) {
+      double Nv = 1;
+#pragma unroll
+      for (int a = 0; a < D; a++) Nv *= ((v >> a) & 1) ? xi[a] : (1 - xi[a]);
+      const double* X = cell_vertex<D>(m, cc, v);
+#pragma unroll
+      for (int d = 0; d < D; d++) x[d] += Nv * X[d];
+    }
+  }
+  xyz[3 * i] = x[0];
+  xyz[3 * i + 1] = x[1];
+  xyz[3 * i + 2] = x[2];
+  owner[i] = cc[0] + m.N[0] * (cc[1] + m.N[1] * cc[2]);
+}
+
+}  // namespace nat
+
+int nat_launch_support_points(nat_ctx* ctx) {
+  nat::MeshDev m;
+  m.D = ctx->D;
+  for (int a = 0; a < 3; a++) {
+    m.N[a] = a < ctx->D ? ctx->desc.ncells[a] : 1;
+    m.periodic[a] = a < ctx->D ? ctx->desc.periodic[a] : 1;
+  }
+  m.cartesian = ctx->desc.cartesian;
+  m.vert = ctx->d_vert;
+  nat::GllTab g;
+  double w[nat::MAXP + 1];
+  nat::gll_points(ctx->p, g.x, w);
+  const int bs = 256;
+  const unsigned grid = (unsigned)((ctx->n_dofs + bs - 1) / bs);
+  if (ctx->D == 3)
+    hipLaunchKernelGGL(nat::k_support_points<3>, dim3(grid), dim3(bs), 0, ctx->stream, m, ctx->p, g, ctx->nfe[0],
+                       ctx->nfe[1], ctx->n_dofs, ctx->d_coords, ctx->d_owner);
+  else
+    hipLaunchKernelGGL(nat::k_support_points<2>, dim3(grid), dim3(bs), 0, ctx->stream, m, ctx->p, g, ctx->nfe[0],
+                       ctx->nfe[1], ctx->n_dofs, ctx->d_coords, ctx->d_owner);
+  NAT_HIP(hipGetLastError());
+  return 0;
+}
+
 int nat_launch_assemble_general(nat_ctx* ctx, const double* d_disp, const double* d_dirw, int* d_cell, double* d_xi,
                                 unsigned* d_flipmask, double* d_bias) {
   nat::MeshDev m;

@@ -14,6 +14,7 @@
 // kernel launchers (k_*.hip)
 int nat_launch_assemble_general(nat_ctx* ctx, const double* d_disp, const double* d_dirw, int* d_cell, double* d_xi,
                                 unsigned* d_flipmask, double* d_bias);
+int nat_launch_support_points(nat_ctx* ctx);
 int nat_launch_wall_compact(nat_ctx* ctx, const int* d_list, const unsigned* d_flipmask, const double* d_bias);
 int nat_launch_assemble_axis(nat_ctx* ctx, const int* d_N, const double* d_vaxis, int maxc, const double* d_disp,
                              const int* d_nshift, double* d_w, int* d_off);
@@ -261,33 +262,6 @@ int check_status(nat_ctx* ctx) {
   if (st & 4) return fail(ctx, NAT_ERR_ARG, "departure point could not be located in its cell");
   if (st & 16) return fail(ctx, NAT_ERR_ARG, "structured departure walk left the axis");
   return 0;
-}
-
-// FE_Q lattice: owning cell and support point of every DoF (SemiLagrangian.cpp:214-229)
-void feq_points(const nat_ctx* c, std::vector<double>& xyz, std::vector<int>& owner) {
-  double gll[MAXP + 1], gw[MAXP + 1];
-  gll_points(c->p, gll, gw);
-  const int64_t n = c->n_dofs;
-  xyz.assign(3 * n, 0.0);
-  owner.assign(n, 0);
-  for (int k = 0; k < c->nfe[2]; k++)
-    for (int j = 0; j < c->nfe[1]; j++)
-      for (int i = 0; i < c->nfe[0]; i++) {
-        const int idx[3] = {i, j, k};
-        int cc[3] = {0, 0, 0}, ml[3] = {0, 0, 0};
-        for (int a = 0; a < c->D; a++) {
-          cc[a] = idx[a] == 0 ? 0 : (idx[a] - 1) / c->p;
-          ml[a] = idx[a] - cc[a] * c->p;
-        }
-        const double xi[3] = {gll[ml[0]], gll[ml[1]], c->D == 3 ? gll[ml[2]] : 0.0};
-        double x[3];
-        map_cell(c, cc, xi, c->desc.cartesian != 0, x);
-        const int64_t d = i + (int64_t)c->nfe[0] * (j + (int64_t)c->nfe[1] * k);
-        xyz[3 * d] = x[0];
-        xyz[3 * d + 1] = x[1];
-        xyz[3 * d + 2] = x[2];
-        owner[d] = cc[0] + c->desc.ncells[0] * (cc[1] + c->desc.ncells[1] * cc[2]);
-      }
 }
 
 // DG nodes: cell-major, local lexicographic
@@ -674,13 +648,10 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
       }
     }
     ctx->n_store = ctx->backend == NAT_BACKEND_STRUCTURED ? (int64_t)ctx->nu[0] * ctx->nu[1] * ctx->nu[2] : ctx->n_dofs;
-    std::vector<double> xyz;
-    std::vector<int> owner;
-    feq_points(ctx, xyz, owner);
-    TRY(alloc(ctx, &ctx->d_coords, xyz.size()));
-    NAT_HIP(hipMemcpy(ctx->d_coords, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice));
-    TRY(alloc(ctx, &ctx->d_owner, owner.size()));
-    NAT_HIP(hipMemcpy(ctx->d_owner, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice));
+    // support points and owner cells of the DoFs, formed on the device
+    TRY(alloc(ctx, &ctx->d_coords, 3 * ctx->n_dofs));
+    TRY(alloc(ctx, &ctx->d_owner, ctx->n_dofs));
+    TRY(nat_launch_support_points(ctx));
   } else {
     const int np = d.order + 1;
     const int64_t ncell = (int64_t)d.ncells[0] * d.ncells[1] * (d.dim == 3 ? d.ncells[2] : 1);
@@ -724,13 +695,13 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
 }
 
 int nat_node_coords(nat_ctx* ctx, double* xyz) {
-  std::vector<double> x;
-  if (ctx->desc.scheme == NAT_SL) {
-    std::vector<int> owner;
-    feq_points(ctx, x, owner);
-  } else {
-    dg_points(ctx, x);
+  if (ctx->desc.scheme == NAT_SL) {  // the support points the assembly walks from
+    NAT_HIP(hipMemcpyAsync(xyz, ctx->d_coords, 3 * ctx->n_dofs * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NAT_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
   }
+  std::vector<double> x;
+  dg_points(ctx, x);
   std::memcpy(xyz, x.data(), x.size() * sizeof(double));
   return 0;
 }
@@ -755,10 +726,7 @@ int nat_set_vertices(nat_ctx* ctx, const double* vertices) {
   NAT_HIP(hipMemcpy(ctx->d_vert, ctx->verts.data(), ctx->verts.size() * sizeof(double), hipMemcpyHostToDevice));
   ctx->h_min = min_vertex_distance(ctx);
   if (d.scheme != NAT_SL) return 0;  // SEDG evaluates the geometry from the vertices in every stage
-  std::vector<double> xyz;
-  std::vector<int> owner;
-  feq_points(ctx, xyz, owner);
-  NAT_HIP(hipMemcpy(ctx->d_coords, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice));
+  TRY(nat_launch_support_points(ctx));
   if (ctx->backend == NAT_BACKEND_STRUCTURED) {
     if (ctx->ax_w) hipFree(ctx->ax_w);
     if (ctx->ax_off) hipFree(ctx->ax_off);
