@@ -68,13 +68,35 @@ def read_traffic(workload):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(prob, seconds):
-    """The oracle (FP64 CPU restatement) on the same workload: assembly untimed, then whole
-    stream + collide steps until about `seconds` of wall time."""
+def host_cpu():
+    """(model name, logical CPUs of the host, threads this process may use)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    n = os.cpu_count() or 1
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = n
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = int(env) if env and env.isdigit() else share
+    return model, n, max(1, min(threads, share))
+
+
+def cpu_oracle_run(prob, steps):
+    """The oracle (FP64 CPU restatement of the reference) on the same workload and initial
+    state: assembly untimed, then `steps` whole stream + collide steps on the host's thread
+    share, timed by wall clock. Returns the cpu_baseline record and the oracle's velocity
+    after `steps` steps (the FP64 reference of the accuracy check)."""
     from natrium_amd.problems import tgv3d_fields
     from oracle import oracle as O
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    model, ncpu, threads = host_cpu()
     O.set_threads(threads)
     o = O.Oracle(prob["stencil"], prob["order"], prob["ncells"], prob["vertices"], cartesian=True,
                  scaling=prob["scaling"], viscosity=prob["viscosity"], cfl=prob["cfl"])
@@ -83,18 +105,16 @@ def cpu_baseline(prob, seconds):
     rho, u = tgv3d_fields(x, prob["cs"])
     o.init(rho, u)
     o.collide()
-    o.steps(1)
-    n = 0
     t0 = time.perf_counter()
-    while True:
-        o.steps(1)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return {"value": o.n * n / el / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
-            "sample": f"{n} FP64 steps (CSR SpMV stream + BGK collide) of the same 16^3 p=4 D3Q27 TGV, "
-                      f"{o.n} FE_Q DoFs, {el:.1f} s wall, OpenMP {threads} threads; assembly untimed"}
+    o.steps(steps)
+    el = time.perf_counter() - t0
+    _, uo, _, _ = o.macro()
+    rec = {"value": o.n * steps / el / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
+           "sample": f"{steps} FP64 steps (CSR SpMV stream + BGK collide) of the same 16^3 p=4 D3Q27 TGV from the "
+                     f"same initial state, {o.n} FE_Q DoFs, {el:.1f} s wall, OpenMP {threads} threads "
+                     f"(the process's CPU share; host has {ncpu} logical CPUs, {model}); assembly untimed",
+           "host_cpu_model": model, "host_logical_cpus": ncpu}
+    return rec, uo
 
 
 def extra_configs(nat, steps_sl=100, steps_err=500, cpu=False):
@@ -191,7 +211,7 @@ def extra_configs(nat, steps_sl=100, steps_err=500, cpu=False):
     if cpu:
         from oracle import oracle as O
 
-        O.set_threads(max(1, min(16, os.cpu_count() or 1)))
+        O.set_threads(host_cpu()[2])
         o = O.Oracle("D3Q19", 3, [24] * 3, v3b, cartesian=False, scaling=s, viscosity=0.01, cfl=0.4)
         t0 = time.perf_counter()
         o.assemble()
@@ -216,25 +236,10 @@ def extra_configs(nat, steps_sl=100, steps_err=500, cpu=False):
 
 
 def serving_requests(n, seed, full):
-    """Seeded serving mix: 2D/3D Taylor-Green or a shock tube on a 2D strip, refinement 3-5,
-    p in {2,3,4,6}, Re log-uniform [10,1600], Ma U[0.05,0.2], steps log-uniform [100,5000].
-    The default (not full) run caps refinement at 4 and steps at 1000 to stay within a minute."""
-    import numpy as np
+    """The seeded serving mix (natrium_amd.problems.serving_mix)."""
+    from natrium_amd.problems import serving_mix
 
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(n):
-        prob = ["TGV2D", "TGV3D", "SHOCK_TUBE"][rng.integers(3)]
-        ref = int(rng.integers(3, 6 if full else 5))
-        p = int([2, 3, 4, 6][rng.integers(4)])
-        st = "D2Q9" if prob != "TGV3D" else ["D3Q19", "D3Q27"][rng.integers(2)]
-        if prob == "TGV3D" and not full:
-            ref = min(ref, 3 if p == 6 else 4)
-        re_ = float(math.exp(rng.uniform(math.log(10), math.log(1600))))
-        ma = float(rng.uniform(0.05, 0.2))
-        steps = int(math.exp(rng.uniform(math.log(100), math.log(5000 if full else 1000))))
-        out.append(dict(problem=prob, stencil=st, refinement=ref, order=p, reynolds=re_, mach=ma, nsteps=steps))
-    return out
+    return serving_mix(n, seed, full)
 
 
 def run_serving(nat, reqs, precision, seed, load=0.8):
@@ -298,7 +303,6 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--error-steps", type=int, default=1000)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-assembly", action="store_true",
                     help="also time the oracle's assembly of the moved C3 operator on the host")
@@ -358,7 +362,8 @@ def main():
     wall64, ms64 = timed_glups(s64, args.steps, args.warmup)
     glups64 = n_unique * args.steps / wall64 / 1e9
 
-    # accuracy: FP32-shifted vs FP64 after error-steps steps from the same initial state
+    # accuracy: FP32-shifted and FP64 GPU runs of error-steps steps from the same initial state,
+    # against each other and against the FP64 oracle run of the cpu_baseline leg
     import numpy as np
 
     a32 = make_solver(nat, prob, "fp32")
@@ -370,9 +375,11 @@ def main():
     a64.step(args.error_steps)
     _, u32, _ = a32.macro()
     _, u64, _ = a64.macro()
-    l2 = float(np.sqrt(((u32 - u64) ** 2).sum() / (u64 ** 2).sum()))
+    rl2 = lambda a, b: float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+    l2 = rl2(u32, u64)
     _, _, e1 = a64.sums()
     t_end = a64.time
+    del a32, a64
 
     # roofline of the fused kernel: every population read once and written once per launch
     bytes_per_launch = n_unique * 2 * Q * 4
@@ -383,7 +390,10 @@ def main():
                 "kernel": "k_sl_struct<3,27,4,float>", "bytes_per_launch": bytes_per_launch,
                 "avg_launch_us": round(avg_launch_s * 1e6, 3)}
 
-    cpu = None if args.no_cpu else cpu_baseline(prob, args.cpu_seconds)
+    cpu, l2_oracle = None, None
+    if not args.no_cpu:
+        cpu, u_oracle = cpu_oracle_run(prob, args.error_steps)
+        l2_oracle = {"fp32": rl2(u32, u_oracle), "fp64": rl2(u64, u_oracle)}
 
     extra = None if args.no_extra else extra_configs(nat, cpu=args.cpu_assembly)
 
@@ -414,6 +424,8 @@ def main():
         "fp32_over_fp64": round(glups32 / glups64, 3),
         "glups_reference_dof_convention": round(n_dofs * args.steps / wall32 / 1e9, 4),
         "fp64_ms_per_step": round(wall64 * 1e3 / args.steps, 5),
+        "l2_err_u_fp32_vs_oracle": l2_oracle["fp32"] if l2_oracle else None,
+        "l2_err_u_fp64_vs_oracle": l2_oracle["fp64"] if l2_oracle else None,
         "l2_err_u_fp32_vs_fp64": l2,
         "l2_err_steps": args.error_steps,
         "ekin_ratio_fp64": e1 / e0,

@@ -123,12 +123,32 @@ int nat_upload_g(nat_ctx* ctx, const double* g);
 int nat_download_g(nat_ctx* ctx, double* g);
 
 /* CFDSolver::collide (CFDSolver.cpp:807-843 -> selectCollision ->
- * CollisionOperator::collideAll, CollisionOperator.h:26-224). Writes rho, u (and T). */
+ * CollisionOperator::collideAll, CollisionOperator.h:26-224) with init == false. Writes rho,
+ * u (and T). The collision arithmetic is that of the fused steps: nat_stream followed by
+ * nat_collide gives the populations of nat_step(ctx, 1) bit for bit (without a stabilizer),
+ * except in FP32 compressible contexts, whose fused step keeps the streamed f and g in FP64
+ * registers where nat_stream stores them in FP32. */
 int nat_collide(nat_ctx* ctx);
 
-/* CFDSolver::stream (CFDSolver.cpp:659-754): SL operator apply, or one low-storage
- * RK5 step of the SEDG operator (RungeKutta5LowStorage.cpp:56-105). */
+/* selectCollision(..., init == true) (CollisionSelection.h:61-67 -> collideAll,
+ * CollisionOperator.h:79-91), the collision of the iterative initialisation
+ * (CFDSolver.cpp:1269-1311): rho is computed and written, the velocity is read from the
+ * stored field instead of computed, and u is not written. u (dim x n_dofs, physical units)
+ * replaces the stored field first; NULL keeps it. Incompressible contexts only. */
+int nat_collide_init(nat_ctx* ctx, const double* u);
+
+/* CFDSolver::stream (CFDSolver.cpp:659-754) alone: for SL, f = A f_old with the wall values
+ * of applyBoundaryConditions (SemiLagrangian.h:150-161; compressible contexts stream g as
+ * well, CompressibleCFDSolver.h:181-314); for SEDG, one low-storage RK5 step
+ * (RungeKutta5LowStorage.cpp:56-105). f_0 is unchanged; time advances by dt. Lets a caller
+ * interpose filter() between stream() and collide() (CFDSolver.cpp:884-891). */
 int nat_stream(nat_ctx* ctx);
+
+/* PseudoEntropicStabilizer::apply (dataprocessors/PseudoEntropicStabilizer.cpp:271-311) as a
+ * pass of its own, for a caller that runs the loop body as nat_stream, nat_collide and the
+ * data processors. D2Q9 and D3Q19 only; other stencils are left unchanged, as in the
+ * reference. (nat_set_stabilizer fuses the same projection into nat_step.) */
+int nat_stabilize(nat_ctx* ctx);
 
 /* nsteps iterations of the CFDSolver::run loop body (CFDSolver.cpp:884-888):
  * stream(); collide(); with stream and collide fused on the device. The caller
@@ -172,6 +192,11 @@ int nat_get_knudsen(nat_ctx* ctx, double* kn);
 int nat_sums(nat_ctx* ctx, double* mass, double* momentum /* dim */, double* ekin);
 
 int nat_sync(nat_ctx* ctx);
+
+/* Launch shape of the structured SL step kernel, no counterpart in the reference: the node
+ * tile (x, y, z extents; z = 1 in 2D) one workgroup of the fused stream + collide kernel
+ * owns on this mesh. */
+int nat_struct_tile(nat_ctx* ctx, int* tile /* 3 */);
 
 /* ---- Output and restart (host side, from fields downloaded through this API) ---- */
 

@@ -53,10 +53,12 @@ EXPORTS = [
     "nat_create", "nat_destroy", "nat_last_error", "nat_n_dofs", "nat_n_stored_nodes", "nat_Q", "nat_dt",
     "nat_tau", "nat_time", "nat_backend", "nat_node_coords", "nat_set_dt", "nat_set_vertices", "nat_set_stabilizer",
     "nat_init_equilibrium",
-    "nat_upload_f", "nat_download_f", "nat_upload_g", "nat_download_g", "nat_collide", "nat_stream",
+    "nat_upload_f", "nat_download_f", "nat_upload_g", "nat_download_g", "nat_collide", "nat_collide_init",
+    "nat_stream", "nat_stabilize",
     "nat_step", "nat_sl_assemble", "nat_sl_bgk_step", "nat_sedg_rk5_step", "nat_compressible_step",
     "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sync",
     "nat_stats", "nat_write_stats", "nat_write_vtu", "nat_checkpoint_write", "nat_checkpoint_read",
+    "nat_struct_tile",
 ]
 
 _lib = None
@@ -84,9 +86,11 @@ def lib():
     for nm in ("nat_dt", "nat_tau", "nat_time"):
         getattr(L, nm).restype = ctypes.c_double
         getattr(L, nm).argtypes = [vp]
-    for nm in ("nat_Q", "nat_backend", "nat_collide", "nat_stream", "nat_sync"):
+    for nm in ("nat_Q", "nat_backend", "nat_collide", "nat_stream", "nat_stabilize", "nat_sync"):
         getattr(L, nm).restype = ctypes.c_int
         getattr(L, nm).argtypes = [vp]
+    L.nat_collide_init.restype = ctypes.c_int
+    L.nat_collide_init.argtypes = [vp, _dp]
     L.nat_node_coords.restype = ctypes.c_int
     L.nat_node_coords.argtypes = [vp, _dp]
     L.nat_set_dt.restype = ctypes.c_int
@@ -125,6 +129,8 @@ def lib():
     L.nat_get_knudsen.argtypes = [vp, _dp]
     L.nat_sums.restype = ctypes.c_int
     L.nat_sums.argtypes = [vp, _dp, _dp, _dp]
+    L.nat_struct_tile.restype = ctypes.c_int
+    L.nat_struct_tile.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     _lib = L
     return L
 
@@ -283,8 +289,22 @@ class Solver:
     def collide(self):
         self._check(lib().nat_collide(self._h))
 
+    def collide_init(self, u=None):
+        """Collision with the velocity read from the stored field (the reference's init == true);
+        u (dim x n, physical units) replaces that field first."""
+        up = None
+        if u is not None:
+            u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.D, self.n)
+            up = _ptr(u)
+        self._check(lib().nat_collide_init(self._h, up))
+
     def stream(self):
+        """CFDSolver::stream alone (SL operator with wall values, or one RK5 step of SEDG)."""
         self._check(lib().nat_stream(self._h))
+
+    def stabilize(self):
+        """The pseudo-entropic stabilizer as a separate data-processor pass (D2Q9 / D3Q19)."""
+        self._check(lib().nat_stabilize(self._h))
 
     def step(self, n=1):
         self._check(lib().nat_step(self._h, int(n)))
@@ -339,6 +359,12 @@ class Solver:
 
     def sync(self):
         self._check(lib().nat_sync(self._h))
+
+    def struct_tile(self):
+        """Node tile (x, y, z) of one workgroup of the structured step kernel on this mesh."""
+        t = (ctypes.c_int * 3)()
+        self._check(lib().nat_struct_tile(self._h, t))
+        return tuple(t)
 
 
 # ----------------------------------------------------------------------------- request server

@@ -6,16 +6,57 @@
 
 namespace nat {
 
-template <int Q, typename T>
+// One thread per node; the same FP64 pair-form collision as the fused step kernels,
+// so stream-only followed by this pass equals a fused step bit for bit. INIT: the
+// reference's initialisation collision (velocity read from the stored field).
+template <int Q, typename T, bool INIT>
 __global__ __launch_bounds__(256) void k_collide(T* f, int64_t stride, int64_t n, CollideArgs<T> ca) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   T fs[Q];
 #pragma unroll
   for (int q = 0; q < Q; q++) fs[q] = f[(int64_t)q * stride + i];
-  bgk_collide<Q, T>(fs, ca, i, true);
+  bgk_collide_store_pairs<Q, T, INIT>(fs, ca, i, true, f, stride);
+}
+
+// The pseudo-entropic stabilizer as a pass of its own (PseudoEntropicStabilizer::apply,
+// dataprocessors/PseudoEntropicStabilizer.cpp:271-311): every node's populations go
+// through the w-orthogonal projection P onto w times the Hermite polynomials of order
+// <= 2, (P f)_i = w_i (rho + e_i.j / cs2 + H2_i : (Pi - rho cs2 I) / (2 cs2^2)). P keeps
+// w rho0, so the shifted FP32 populations are projected as they are. D2Q9 / D3Q19 only.
+template <int Q, typename T>
+__global__ __launch_bounds__(256) void k_stabilize(T* f, int64_t stride, int64_t n) {
+  using L = Lattice<Q>;
+  constexpr int D = L::D;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double fd[Q];
 #pragma unroll
-  for (int q = 0; q < Q; q++) f[(int64_t)q * stride + i] = fs[q];
+  for (int q = 0; q < Q; q++) fd[q] = (double)f[(int64_t)q * stride + i];
+  double m0 = 0, j[3] = {0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    m0 += fd[q];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (L::e[q][d] > 0) j[d] += fd[q];
+      if (L::e[q][d] < 0) j[d] -= fd[q];
+    }
+  }
+  // second-order coefficients 9/2 (Pi - m0 I / 3), as the collision's non-equilibrium
+  // projection at u = 0 and omega = 0
+  const double zero[3] = {0, 0, 0};
+  const StabPi<Q> m = stab_neq<Q>(stab_moments<Q>(fd), m0, 0.0, zero, 0.0);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    double ej = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      if (L::e[q][d] > 0) ej += j[d];
+      if (L::e[q][d] < 0) ej -= j[d];
+    }
+    f[(int64_t)q * stride + i] = (T)(L::w[q] * (m0 + 3.0 * ej) + stab_term<Q>(m, q));
+  }
 }
 
 // Layouts of the macroscopic fields for the quadrature sums.
@@ -31,10 +72,23 @@ struct SumArgs {
   double gllw[MAXP + 1];
 };
 
-// one thread per (cell, local node); block partial sums -> FP64 atomics
+// Sum of a double over the 64 lanes of a wave by a fixed butterfly (every lane ends
+// with the same total, formed in the same order on every run).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+constexpr int SUM_BS = 256;  // threads per block of both passes
+
+// Pass 1: one thread per (cell, local node); each block writes its five partial sums
+// (mass, momentum, 2 E_kin) to partial[5 * block]. Wave sums by shuffles, then the
+// block's waves added in wave order: no atomics, so the result does not depend on the
+// order in which blocks run.
 template <typename T>
-__global__ __launch_bounds__(256) void k_sums(SumArgs s, const T* __restrict__ rho, const T* __restrict__ u,
-                                              double* __restrict__ out /* mass, mom[3], 2*ekin */) {
+__global__ __launch_bounds__(SUM_BS) void k_sums(SumArgs s, const T* __restrict__ rho, const T* __restrict__ u,
+                                                 double* __restrict__ partial) {
   const int np = s.p + 1;
   const int nl = s.m.D == 3 ? np * np * np : np * np;
   const int64_t ncell = (int64_t)s.m.N[0] * s.m.N[1] * (s.m.D == 3 ? s.m.N[2] : 1);
@@ -77,54 +131,103 @@ __global__ __launch_bounds__(256) void k_sums(SumArgs s, const T* __restrict__ r
     v[0] = r * jxw;
     v[4] = r * uu * jxw;
   }
-  __shared__ double red[5][256];
-  for (int k = 0; k < 5; k++) red[k][threadIdx.x] = v[k];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int k = 0; k < 5; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
-    __syncthreads();
+  __shared__ double red[SUM_BS / 64][5];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const double w = wave_sum(v[k]);
+    if (lane == 0) red[wv][k] = w;
   }
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 5; k++) atomicAdd(out + k, red[k][0]);
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double a = red[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < SUM_BS / 64; w++) a += red[w][threadIdx.x];
+    partial[(int64_t)blockIdx.x * 5 + threadIdx.x] = a;
+  }
+}
+
+// Pass 2: one block adds the nb partial sums: thread t takes blocks t, t + 256, ... in
+// order, then the same wave and block reduction as pass 1.
+__global__ __launch_bounds__(SUM_BS) void k_sums_final(const double* __restrict__ partial, int64_t nb,
+                                                       double* __restrict__ out) {
+  double v[5] = {0, 0, 0, 0, 0};
+  for (int64_t b = threadIdx.x; b < nb; b += SUM_BS)
+#pragma unroll
+    for (int k = 0; k < 5; k++) v[k] += partial[b * 5 + k];
+  __shared__ double red[SUM_BS / 64][5];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const double w = wave_sum(v[k]);
+    if (lane == 0) red[wv][k] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double a = red[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < SUM_BS / 64; w++) a += red[w][threadIdx.x];
+    out[threadIdx.x] = a;
+  }
 }
 
 }  // namespace nat
 
-int nat_launch_collide(nat_ctx* ctx) {
+template <typename T>
+static int launch_collide_typed(nat_ctx* ctx, bool init) {
   using namespace nat;
   const int bs = 256;
   const unsigned grid = (unsigned)((ctx->n_store + bs - 1) / bs);
+  CollideArgs<T> ca;
+  ca.omega = (T)(1.0 / ctx->tau);
+  ca.omega64 = 1.0 / ctx->tau;
+  ca.rho0 = (T)(ctx->fp32 ? (ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0) : 0.0);
+  ca.scaling = ctx->st.s;
+  ca.rho = (T*)ctx->rho;
+  ca.u = (T*)ctx->u;
+  ca.mstride = ctx->n_store;
+  ca.status = ctx->d_status;
+  ca.stab = 0;
+  T* f = (T*)ctx->f[ctx->cur];
+#define NAT_KC(QQ)                                                                                              \
+  if (ctx->Q == QQ) {                                                                                          \
+    if (init)                                                                                                  \
+      hipLaunchKernelGGL((k_collide<QQ, T, true>), dim3(grid), dim3(bs), 0, ctx->stream, f, ctx->stride,       \
+                         ctx->n_store, ca);                                                                    \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_collide<QQ, T, false>), dim3(grid), dim3(bs), 0, ctx->stream, f, ctx->stride,      \
+                         ctx->n_store, ca);                                                                    \
+    NAT_HIP(hipGetLastError());                                                                                \
+    return 0;                                                                                                  \
+  }
+  NAT_KC(9)
+  NAT_KC(19)
+  NAT_KC(27)
+#undef NAT_KC
+  ctx->err = "collision not available for this stencil";
+  return NAT_ERR_UNSUPPORTED;
+}
+
+int nat_launch_collide(nat_ctx* ctx, int init) {
+  return ctx->fp32 ? launch_collide_typed<float>(ctx, init != 0) : launch_collide_typed<double>(ctx, init != 0);
+}
+
+int nat_launch_stabilize(nat_ctx* ctx) {
+  using namespace nat;
+  if (ctx->Q != 9 && ctx->Q != 19) return 0;  // the reference leaves other stencils alone
+  const int bs = 256;
+  const unsigned grid = (unsigned)((ctx->n_store + bs - 1) / bs);
+  void* f = ctx->f[ctx->cur];
   if (ctx->fp32) {
-    CollideArgs<float> ca;
-    ca.omega = (float)(1.0 / ctx->tau);
-    ca.omega64 = 1.0 / ctx->tau;
-    ca.rho0 = (float)(ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0);
-    ca.scaling = ctx->st.s;
-    ca.rho = (float*)ctx->rho;
-    ca.u = (float*)ctx->u;
-    ca.mstride = ctx->n_store;
-    ca.status = ctx->d_status;
-    float* f = (float*)ctx->f[ctx->cur];
-    if (ctx->Q == 9) hipLaunchKernelGGL((k_collide<9, float>), dim3(grid), dim3(bs), 0, ctx->stream, f, ctx->stride, ctx->n_store, ca);
-    else if (ctx->Q == 19) hipLaunchKernelGGL((k_collide<19, float>), dim3(grid), dim3(bs), 0, ctx->stream, f, ctx->stride, ctx->n_store, ca);
-    else if (ctx->Q == 27) hipLaunchKernelGGL((k_collide<27, float>), dim3(grid), dim3(bs), 0, ctx->stream, f, ctx->stride, ctx->n_store, ca);
-    else { ctx->err = "collision not available for this stencil"; return NAT_ERR_UNSUPPORTED; }
+    if (ctx->Q == 9)
+      hipLaunchKernelGGL((k_stabilize<9, float>), dim3(grid), dim3(bs), 0, ctx->stream, (float*)f, ctx->stride, ctx->n_store);
+    else
+      hipLaunchKernelGGL((k_stabilize<19, float>), dim3(grid), dim3(bs), 0, ctx->stream, (float*)f, ctx->stride, ctx->n_store);
   } else {
-    CollideArgs<double> ca;
-    ca.omega = 1.0 / ctx->tau;
-    ca.omega64 = 1.0 / ctx->tau;
-    ca.rho0 = 0.0;
-    ca.scaling = ctx->st.s;
-    ca.rho = (double*)ctx->rho;
-    ca.u = (double*)ctx->u;
-    ca.mstride = ctx->n_store;
-    ca.status = ctx->d_status;
-    double* f = (double*)ctx->f[ctx->cur];
-    if (ctx->Q == 9) hipLaunchKernelGGL((k_collide<9, double>), dim3(grid), dim3(bs), 0, ctx->stream, f, ctx->stride, ctx->n_store, ca);
-    else if (ctx->Q == 19) hipLaunchKernelGGL((k_collide<19, double>), dim3(grid), dim3(bs), 0, ctx->stream, f, ctx->stride, ctx->n_store, ca);
-    else if (ctx->Q == 27) hipLaunchKernelGGL((k_collide<27, double>), dim3(grid), dim3(bs), 0, ctx->stream, f, ctx->stride, ctx->n_store, ca);
-    else { ctx->err = "collision not available for this stencil"; return NAT_ERR_UNSUPPORTED; }
+    if (ctx->Q == 9)
+      hipLaunchKernelGGL((k_stabilize<9, double>), dim3(grid), dim3(bs), 0, ctx->stream, (double*)f, ctx->stride, ctx->n_store);
+    else
+      hipLaunchKernelGGL((k_stabilize<19, double>), dim3(grid), dim3(bs), 0, ctx->stream, (double*)f, ctx->stride, ctx->n_store);
   }
   NAT_HIP(hipGetLastError());
   return 0;
@@ -147,15 +250,23 @@ int nat_launch_sums(nat_ctx* ctx, int layout) {
   gll_points(ctx->p, s.gll, s.gllw);
   const int np = ctx->p + 1;
   const int64_t total = (int64_t)s.m.N[0] * s.m.N[1] * s.m.N[2] * (ctx->D == 3 ? np * np * np : np * np);
-  NAT_HIP(hipMemsetAsync(ctx->d_sums, 0, 5 * sizeof(double), ctx->stream));
-  const int bs = 256;
-  const unsigned grid = (unsigned)((total + bs - 1) / bs);
+  const int64_t grid = (total + SUM_BS - 1) / SUM_BS;
+  if (grid > ctx->n_partials) {
+    if (ctx->d_partials) hipFree(ctx->d_partials);
+    ctx->d_partials = nullptr;
+    ctx->n_partials = 0;
+    NAT_HIP(hipMalloc((void**)&ctx->d_partials, (size_t)grid * 5 * sizeof(double)));
+    ctx->n_partials = grid;
+  }
   if (ctx->fp32)
-    hipLaunchKernelGGL((k_sums<float>), dim3(grid), dim3(bs), 0, ctx->stream, s, (const float*)ctx->rho,
-                       (const float*)ctx->u, ctx->d_sums);
+    hipLaunchKernelGGL((k_sums<float>), dim3((unsigned)grid), dim3(SUM_BS), 0, ctx->stream, s,
+                       (const float*)ctx->rho, (const float*)ctx->u, ctx->d_partials);
   else
-    hipLaunchKernelGGL((k_sums<double>), dim3(grid), dim3(bs), 0, ctx->stream, s, (const double*)ctx->rho,
-                       (const double*)ctx->u, ctx->d_sums);
+    hipLaunchKernelGGL((k_sums<double>), dim3((unsigned)grid), dim3(SUM_BS), 0, ctx->stream, s,
+                       (const double*)ctx->rho, (const double*)ctx->u, ctx->d_partials);
+  NAT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(SUM_BS), 0, ctx->stream, (const double*)ctx->d_partials, grid,
+                     ctx->d_sums);
   NAT_HIP(hipGetLastError());
   return 0;
 }

@@ -62,6 +62,7 @@ struct CompArgs {
   double* mask;
   int64_t mstride;
   int* status;
+  int stream_only;  // CompressibleCFDSolver::stream alone: store the streamed f and g, no collision
 };
 
 // Quartic equilibrium of direction q for the node scalars (2D). The node factors
@@ -315,10 +316,17 @@ __global__ __launch_bounds__(comp_waves<PR>() * 64) void k_sl_comp_wide(CompArgs
     widx = a.wall_idx[i];
     if (widx >= 0) wmask = a.wall_flip[widx];
   }
+  // stream only (uniform over the grid): each wave stores the directions it streamed
+  const bool so = ST && a.stream_only;
   if (live) {
     if (w == 0) {
-      sf[0][lane] = (double)a.fin[i] + tab[CT_W] * a.rho0;
-      sg[0][lane] = (double)a.gin[i] + tab[CT_W] * a.gshift;
+      if (so) {
+        a.fout[i] = a.fin[i];
+        a.gout[i] = a.gin[i];
+      } else {
+        sf[0][lane] = (double)a.fin[i] + tab[CT_W] * a.rho0;
+        sg[0][lane] = (double)a.gin[i] + tab[CT_W] * a.gshift;
+      }
     }
     for (int al = 1 + w; al < Q; al += CW) {
       double fo, go;
@@ -328,10 +336,16 @@ __global__ __launch_bounds__(comp_waves<PR>() * 64) void k_sl_comp_wide(CompArgs
         fo = (double)a.fin[(int64_t)al * a.stride + i] + tab[al * CTAB + CT_W] * a.rho0;
         go = (double)a.gin[(int64_t)al * a.stride + i] + tab[al * CTAB + CT_W] * a.gshift;
       }
-      sf[al][lane] = fo;
-      sg[al][lane] = go;
+      if (so) {
+        a.fout[(int64_t)al * a.stride + i] = (T)(fo - tab[al * CTAB + CT_W] * a.rho0);
+        a.gout[(int64_t)al * a.stride + i] = (T)(go - tab[al * CTAB + CT_W] * a.gshift);
+      } else {
+        sf[al][lane] = fo;
+        sg[al][lane] = go;
+      }
     }
   }
+  if (so) return;
   __syncthreads();
   const auto fl = [&](int q) { return sf[q][lane]; };
   const auto gl = [&](int q) { return sg[q][lane]; };
@@ -484,13 +498,16 @@ static void fill_comp(nat_ctx* ctx, CompArgs<T>& a, int src, int dst) {
   a.mask = ctx->mask;
   a.mstride = ctx->n_store;
   a.status = ctx->d_status;
+  a.stream_only = 0;
 }
 
 template <typename T>
-static int step_comp(nat_ctx* ctx, int write_macro) {
+static int step_comp(nat_ctx* ctx, int mode) {
   CompArgs<T> a;
   const int src = ctx->cur, dst = 1 - ctx->cur;
   fill_comp<T>(ctx, a, src, dst);
+  a.stream_only = (mode & STEP_STREAM_ONLY) ? 1 : 0;
+  const int write_macro = (mode & STEP_MACRO) ? 1 : 0;
   const unsigned grid = (unsigned)((a.n + 63) / 64);
   const double* tab = ctx->comp_tab;
   const dim3 blk(comp_waves<true>() * 64), blk0(comp_waves<false>() * 64);

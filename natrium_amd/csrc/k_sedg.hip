@@ -207,7 +207,7 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
     }
   });
   if constexpr (COL) {
-    if (active) bgk_collide_store<Q, T>(fs, ca, node, write_macro != 0, s.fout, s.stride);
+    if (active) bgk_collide_store_pairs<Q, T>(fs, ca, node, write_macro != 0, s.fout, s.stride);
   }
 }
 

@@ -258,7 +258,12 @@ __global__ __launch_bounds__((GeneralShape<Q, T>::bs)) __attribute__((amdgpu_wav
 #pragma unroll
     for (int al = 1; al < Q; al++) fs[al] = gen_direction<D, Q, P, T>(a, i, al, widx, wmask);
   }
-  bgk_collide_store<Q, T>(fs, ca, i, write_macro != 0, a.fout, a.stride);
+  if (write_macro & STEP_STREAM_ONLY) {  // the SL stream with its wall values (CFDSolver::stream)
+#pragma unroll
+    for (int q = 0; q < Q; q++) a.fout[(int64_t)q * a.stride + i] = fs[q];
+    return;
+  }
+  bgk_collide_store_pairs<Q, T>(fs, ca, i, (write_macro & STEP_MACRO) != 0, a.fout, a.stride);
 }
 
 template <int D, int Q, int P, typename T>

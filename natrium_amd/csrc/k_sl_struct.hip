@@ -462,11 +462,24 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
   });
 
   if (!active) return;
-  bgk_collide_store_pairs<Q, T>(fs, ca, node, write_macro != 0, a.fout, a.stride);
+  if (write_macro & STEP_STREAM_ONLY) {  // the SL stream alone (CFDSolver::stream)
+#pragma unroll
+    for (int q = 0; q < Q; q++) a.fout[(int64_t)q * a.stride + node] = fs[q];
+    return;
+  }
+  bgk_collide_store_pairs<Q, T>(fs, ca, node, (write_macro & STEP_MACRO) != 0, a.fout, a.stride);
 }
 
+// query != null: report the tile (TX, TY, TZ) instead of launching
 template <int D, int Q, int P, typename T, int TX, int TY, int TZ, bool SYM_ONLY = false>
-static int launch_struct(nat_ctx* ctx, StructArgs<T> a, const CollideArgs<T>& ca, int write_macro) {
+static int launch_struct(nat_ctx* ctx, StructArgs<T> a, const CollideArgs<T>& ca, int write_macro,
+                         int* query) {
+  if (query) {
+    query[0] = TX;
+    query[1] = TY;
+    query[2] = TZ;
+    return 0;
+  }
   a.tiles[0] = (a.n[0] + TX - 1) / TX;
   a.tiles[1] = (a.n[1] + TY - 1) / TY;
   a.tiles[2] = D == 3 ? (a.n[2] + TZ - 1) / TZ : 1;
@@ -492,24 +505,24 @@ static int launch_struct(nat_ctx* ctx, StructArgs<T> a, const CollideArgs<T>& ca
 // the periodic boxes whose 1-D tables are symmetric (all uniform boxes) at p = 2, 3, 4; at
 // p = 6 a small tile is one cell plus its halo and measured slower.
 template <int D, int Q, int P, typename T, int TX, int TY, int TZ, int SX, int SY, int SZ>
-static int launch_struct_sized(nat_ctx* ctx, const StructArgs<T>& a, const CollideArgs<T>& ca, int wm) {
+static int launch_struct_sized(nat_ctx* ctx, const StructArgs<T>& a, const CollideArgs<T>& ca, int wm, int* query) {
   const int64_t nb = (int64_t)((a.n[0] + TX - 1) / TX) * ((a.n[1] + TY - 1) / TY) * (D == 3 ? (a.n[2] + TZ - 1) / TZ : 1);
-  if (ctx->ax_sym && nb < 256) return launch_struct<D, Q, P, T, SX, SY, SZ, true>(ctx, a, ca, wm);
-  return launch_struct<D, Q, P, T, TX, TY, TZ>(ctx, a, ca, wm);
+  if (ctx->ax_sym && nb < 256) return launch_struct<D, Q, P, T, SX, SY, SZ, true>(ctx, a, ca, wm, query);
+  return launch_struct<D, Q, P, T, TX, TY, TZ>(ctx, a, ca, wm, query);
 }
 
 template <typename T>
-static int dispatch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideArgs<T>& ca, int wm) {
+static int dispatch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideArgs<T>& ca, int wm, int* query) {
   const int Q = ctx->Q, P = ctx->p;
 #define NAT_S3(QQ, PP, TX, TY, TZ) \
-  if (Q == QQ && P == PP) return launch_struct<3, QQ, PP, T, TX, TY, TZ>(ctx, a, ca, wm);
+  if (Q == QQ && P == PP) return launch_struct<3, QQ, PP, T, TX, TY, TZ>(ctx, a, ca, wm, query);
 #define NAT_S3S(QQ, PP, TX, TY, TZ, SX, SY, SZ) \
-  if (Q == QQ && P == PP) return launch_struct_sized<3, QQ, PP, T, TX, TY, TZ, SX, SY, SZ>(ctx, a, ca, wm);
+  if (Q == QQ && P == PP) return launch_struct_sized<3, QQ, PP, T, TX, TY, TZ, SX, SY, SZ>(ctx, a, ca, wm, query);
 #define NAT_S3X(...) NAT_S3S(__VA_ARGS__)
 #define NAT_S2(QQ, PP, TX, TY) \
-  if (Q == QQ && P == PP) return launch_struct<2, QQ, PP, T, TX, TY, 1>(ctx, a, ca, wm);
+  if (Q == QQ && P == PP) return launch_struct<2, QQ, PP, T, TX, TY, 1>(ctx, a, ca, wm, query);
 #define NAT_S2S(QQ, PP, TX, TY, SX, SY) \
-  if (Q == QQ && P == PP) return launch_struct_sized<2, QQ, PP, T, TX, TY, 1, SX, SY, 1>(ctx, a, ca, wm);
+  if (Q == QQ && P == PP) return launch_struct_sized<2, QQ, PP, T, TX, TY, 1, SX, SY, 1>(ctx, a, ca, wm, query);
   if (ctx->D == 3) {
     NAT_S3(27, 1, 16, 8, 8)
     NAT_S3S(27, 2, 16, 8, 8, 8, 8, 4)
@@ -538,7 +551,11 @@ static int dispatch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideAr
 }
 
 template <typename T>
-static int launch_typed(nat_ctx* ctx, int write_macro) {
+static int launch_typed(nat_ctx* ctx, int write_macro, int* query = nullptr) {
+  if (!ctx->ax_w || !ctx->ax_off) {
+    ctx->err = "structured backend: no 1-D operator tables (the last assembly failed)";
+    return NAT_ERR_STATE;
+  }
   const int src = ctx->cur, dst = 1 - ctx->cur;
   StructArgs<T> a;
   a.fin = (const T*)ctx->f[src];
@@ -561,8 +578,8 @@ static int launch_typed(nat_ctx* ctx, int write_macro) {
   ca.mstride = ctx->n_store;
   ca.status = ctx->d_status;
   ca.stab = ctx->stab_on();
-  int rc = dispatch_struct<T>(ctx, a, ca, write_macro);
-  if (rc == 0) ctx->cur = dst;
+  int rc = dispatch_struct<T>(ctx, a, ca, write_macro, query);
+  if (rc == 0 && !query) ctx->cur = dst;
   return rc;
 }
 
@@ -570,4 +587,8 @@ static int launch_typed(nat_ctx* ctx, int write_macro) {
 
 int nat_launch_sl_struct(nat_ctx* ctx, int write_macro) {
   return ctx->fp32 ? nat::launch_typed<float>(ctx, write_macro) : nat::launch_typed<double>(ctx, write_macro);
+}
+
+int nat_launch_sl_struct_tile(nat_ctx* ctx, int* tile) {
+  return ctx->fp32 ? nat::launch_typed<float>(ctx, 0, tile) : nat::launch_typed<double>(ctx, 0, tile);
 }

@@ -90,173 +90,6 @@ __device__ __forceinline__ double stab_term(const StabPi<Q>& m, int i) {
   return L::w[i] * (v - tr / 3.0);
 }
 
-// cs2 = 1/3 in unscaled units for D2Q9 / D3Q19 / D3Q27
-template <int Q, typename T>
-__device__ __forceinline__ void bgk_collide(T (&f)[Q], const CollideArgs<T>& ca, int64_t node, bool write_macro) {
-  using L = Lattice<Q>;
-  constexpr int D = L::D;
-  T drho = 0;
-#pragma unroll
-  for (int i = 0; i < Q; i++) drho += f[i];
-  T j[3] = {0, 0, 0};
-#pragma unroll
-  for (int i = 0; i < Q; i++) {
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      if (L::e[i][d] > 0) j[d] += f[i];
-      if (L::e[i][d] < 0) j[d] -= f[i];
-    }
-  }
-  const T rho = ca.rho0 + drho;
-  if (!(rho >= T(1e-10))) atomicOr(ca.status, 1);
-  const T inv = T(1) / rho;
-  T u[3];
-#pragma unroll
-  for (int d = 0; d < D; d++) u[d] = j[d] * inv;
-  T uu = 0;
-#pragma unroll
-  for (int d = 0; d < D; d++) uu += u[d] * u[d];
-  const T c_uu = T(1.5) * uu;
-  const T om = ca.omega;
-#pragma unroll
-  for (int i = 0; i < Q; i++) {
-    T ue = 0;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      if (L::e[i][d] > 0) ue += u[d];
-      if (L::e[i][d] < 0) ue -= u[d];
-    }
-    ue *= T(3);
-    const T feq = T(L::w[i]) * (drho + rho * (ue * (T(1) + T(0.5) * ue) - c_uu));
-    f[i] -= om * (f[i] - feq);
-  }
-  if (write_macro && ca.rho) {
-    ca.rho[node] = rho;
-#pragma unroll
-    for (int d = 0; d < D; d++) ca.u[d * ca.mstride + node] = u[d] * T(ca.scaling);
-  }
-}
-
-// Same collision with FP64 arithmetic on values held in FP64 registers, for any
-// storage type T (FP32 storage keeps the shift by rho0; FP64 storage has rho0 = 0).
-template <int Q, typename T>
-__device__ __forceinline__ void bgk_collide_f64(double (&f)[Q], const CollideArgs<T>& ca, int64_t node,
-                                                bool write_macro) {
-  using L = Lattice<Q>;
-  constexpr int D = L::D;
-  double drho = 0;
-#pragma unroll
-  for (int i = 0; i < Q; i++) drho += f[i];
-  double j[3] = {0, 0, 0};
-#pragma unroll
-  for (int i = 0; i < Q; i++) {
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      if (L::e[i][d] > 0) j[d] += f[i];
-      if (L::e[i][d] < 0) j[d] -= f[i];
-    }
-  }
-  const double rho = (double)ca.rho0 + drho;
-  if (!(rho >= 1e-10)) atomicOr(ca.status, 1);
-  const double inv = 1.0 / rho;
-  double u[3];
-#pragma unroll
-  for (int d = 0; d < D; d++) u[d] = j[d] * inv;
-  double uu = 0;
-#pragma unroll
-  for (int d = 0; d < D; d++) uu += u[d] * u[d];
-  const double c_uu = 1.5 * uu;
-  const double om = ca.omega64;
-#pragma unroll
-  for (int i = 0; i < Q; i++) {
-    double ue = 0;
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      if (L::e[i][d] > 0) ue += u[d];
-      if (L::e[i][d] < 0) ue -= u[d];
-    }
-    ue *= 3.0;
-    const double feq = L::w[i] * (drho + rho * (ue * (1.0 + 0.5 * ue) - c_uu));
-    f[i] -= om * (f[i] - feq);
-  }
-  if (write_macro && ca.rho) {
-    ca.rho[node] = (T)rho;
-#pragma unroll
-    for (int d = 0; d < D; d++) ca.u[d * ca.mstride + node] = (T)(u[d] * ca.scaling);
-  }
-}
-
-// FP64 collision of populations held in storage precision, storing each relaxed
-// population as soon as it is formed (f~ stays in T registers; only the moments
-// and one population at a time live in FP64).
-template <int Q, typename T>
-__device__ __forceinline__ void bgk_collide_store(const T (&f)[Q], const CollideArgs<T>& ca, int64_t node,
-                                                  bool write_macro, T* __restrict__ fout, int64_t stride) {
-  using L = Lattice<Q>;
-  constexpr int D = L::D;
-  double drho = 0;
-#pragma unroll
-  for (int i = 0; i < Q; i++) drho += (double)f[i];
-  double j[3] = {0, 0, 0};
-#pragma unroll
-  for (int i = 0; i < Q; i++) {
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-      if (L::e[i][d] > 0) j[d] += (double)f[i];
-      if (L::e[i][d] < 0) j[d] -= (double)f[i];
-    }
-  }
-  const double rho = (double)ca.rho0 + drho;
-  if (!(rho >= 1e-10)) atomicOr(ca.status, 1);
-  const double inv = 1.0 / rho;
-  double u[3];
-#pragma unroll
-  for (int d = 0; d < D; d++) u[d] = j[d] * inv;
-  double uu = 0;
-#pragma unroll
-  for (int d = 0; d < D; d++) uu += u[d] * u[d];
-  const double c_uu = 1.5 * uu;
-  const double om = ca.omega64;
-  if ((Q == 9 || Q == 19) && ca.stab) {
-    double fd[Q];
-#pragma unroll
-    for (int i = 0; i < Q; i++) fd[i] = (double)f[i];
-    double u3[3] = {u[0], u[1], D == 3 ? u[2] : 0.0};
-    const StabPi<Q> pn = stab_neq<Q>(stab_moments<Q>(fd), drho, rho, u3, om);
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-      double ue = 0;
-#pragma unroll
-      for (int d = 0; d < D; d++) {
-        if (L::e[i][d] > 0) ue += u[d];
-        if (L::e[i][d] < 0) ue -= u[d];
-      }
-      ue *= 3.0;
-      const double feq = L::w[i] * (drho + rho * (ue * (1.0 + 0.5 * ue) - c_uu));
-      fout[(int64_t)i * stride + node] = (T)(feq + stab_term<Q>(pn, i));
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < Q; i++) {
-      double ue = 0;
-#pragma unroll
-      for (int d = 0; d < D; d++) {
-        if (L::e[i][d] > 0) ue += u[d];
-        if (L::e[i][d] < 0) ue -= u[d];
-      }
-      ue *= 3.0;
-      const double feq = L::w[i] * (drho + rho * (ue * (1.0 + 0.5 * ue) - c_uu));
-      const double fi = (double)f[i];
-      fout[(int64_t)i * stride + node] = (T)(fi - om * (fi - feq));
-    }
-  }
-  if (write_macro && ca.rho) {
-    ca.rho[node] = (T)rho;
-#pragma unroll
-    for (int d = 0; d < D; d++) ca.u[d * ca.mstride + node] = (T)(u[d] * ca.scaling);
-  }
-}
-
 // Opposite direction of i in the integer velocity table.
 template <int Q>
 constexpr int opposite(int i) {
@@ -267,11 +100,17 @@ constexpr int opposite(int i) {
   return -1;
 }
 
-// bgk_collide_store with the equilibrium of each opposite pair (i, i') formed from
+// FP64 BGK collision of populations held in storage precision, every relaxed population
+// stored as soon as it is formed. The equilibrium of each opposite pair (i, i') comes from
 // shared even and odd parts: f_eq(i/i') = w (E +- O), E = drho + rho (ue^2/2 - 3u.u/2),
-// O = rho ue (same values as the direction-by-direction form up to FP64 rounding).
-// Stores go through a uniform direction base and a 32-bit byte offset.
-template <int Q, typename T>
+// O = rho ue (the direction-by-direction form of Equilibria.h:66-83 up to FP64 rounding).
+// Every step kernel and the collision-only pass use this one function, so their results
+// agree bit for bit. Stores go through a uniform direction base and a 32-bit byte offset.
+//
+// INIT: the collision of the reference's iterative initialisation
+// (collision_advanced/CollisionOperator.h:79-91, inInitializationProcedure): the velocity
+// is read from the stored field (u / scaling) instead of computed, and only rho is written.
+template <int Q, typename T, bool INIT = false>
 __device__ __forceinline__ void bgk_collide_store_pairs(const T (&f)[Q], const CollideArgs<T>& ca, int64_t node,
                                                         bool write_macro, T* __restrict__ fout, int64_t stride) {
   using L = Lattice<Q>;
@@ -297,7 +136,10 @@ __device__ __forceinline__ void bgk_collide_store_pairs(const T (&f)[Q], const C
   double u[3], u3[3];
 #pragma unroll
   for (int d = 0; d < D; d++) {
-    u[d] = j[d] * inv;
+    if constexpr (INIT)
+      u[d] = (double)ca.u[d * ca.mstride + node] / ca.scaling;
+    else
+      u[d] = j[d] * inv;
     u3[d] = 3.0 * u[d];
   }
   double uu = 0;
@@ -368,8 +210,10 @@ __device__ __forceinline__ void bgk_collide_store_pairs(const T (&f)[Q], const C
   }
   if (write_macro && ca.rho) {
     ca.rho[node] = (T)rho;
+    if constexpr (!INIT) {
 #pragma unroll
-    for (int d = 0; d < D; d++) ca.u[d * ca.mstride + node] = (T)(u[d] * ca.scaling);
+      for (int d = 0; d < D; d++) ca.u[d * ca.mstride + node] = (T)(u[d] * ca.scaling);
+    }
   }
 }
 

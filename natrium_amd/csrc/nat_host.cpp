@@ -20,8 +20,10 @@ int nat_launch_assemble_axis(nat_ctx* ctx, const int* d_N, const double* d_vaxis
                              const int* d_nshift, double* d_w, int* d_off);
 int nat_launch_records_f32(nat_ctx* ctx, int64_t nrec);
 int nat_launch_sl_struct(nat_ctx* ctx, int write_macro);
+int nat_launch_sl_struct_tile(nat_ctx* ctx, int* tile);
 int nat_launch_sl_general(nat_ctx* ctx, int write_macro);
-int nat_launch_collide(nat_ctx* ctx);
+int nat_launch_collide(nat_ctx* ctx, int init);
+int nat_launch_stabilize(nat_ctx* ctx);
 int nat_launch_sums(nat_ctx* ctx, int layout);
 int nat_sedg_setup(nat_ctx* ctx);
 int nat_sedg_step(nat_ctx* ctx, int collide_after, int write_macro);
@@ -253,9 +255,10 @@ int fail(nat_ctx* ctx, int code, const std::string& msg) {
   } while (0)
 
 int check_status(nat_ctx* ctx) {
-  int st = 0;
-  NAT_HIP(hipMemcpyAsync(&st, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  // the copy lands in pinned host memory: no staging buffer, one wait for the whole call
+  NAT_HIP(hipMemcpyAsync(ctx->h_status, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   NAT_HIP(hipStreamSynchronize(ctx->stream));
+  const int st = *ctx->h_status;
   if (st) NAT_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
   if (st & 1) return fail(ctx, NAT_ERR_DENSITY, "Densities too small (< 1e-10) for collisions. Decrease time step size.");
   if (st & 2) return fail(ctx, NAT_ERR_UNSUPPORTED, "departure path reached a wall but no wall table was built");
@@ -295,9 +298,53 @@ inline int64_t store_index(const nat_ctx* c, int64_t dof) {
   return ii + (int64_t)c->nu[0] * (jj + (int64_t)c->nu[1] * kk);
 }
 
-// Structured 1-D tables on the GPU (FP64), then host checks: offsets within one
-// cell layer and no tensor-product weight the reference would drop (|w| < 1e-10).
-int setup_structured_tables(nat_ctx* ctx, bool* ok) {
+// Device scratch freed when the set-up function returns, on every path.
+struct DevScratch {
+  std::vector<void*> bufs;
+  ~DevScratch() {
+    for (void* b : bufs)
+      if (b) hipFree(b);
+  }
+  template <typename T>
+  int get(nat_ctx* ctx, T** p, size_t count) {
+    int rc = alloc(ctx, p, count);
+    if (!rc) bufs.push_back(*p);
+    return rc;
+  }
+};
+
+// Structured 1-D tables of a candidate operator. They are built beside the tables in
+// use and only installed (commit_structured_tables) once every check has passed, so a
+// time step or mesh the structured backend cannot take leaves the current operator intact.
+struct StructTables {
+  double* w = nullptr;  // [3][MAXSHIFT][MAXP][MAXP+1] FP64 weights
+  int* off = nullptr;   // [3][MAXSHIFT][MAXP] source cell offsets
+  bool sym = false;
+  int off_lo[3][MAXSHIFT] = {}, off_hi[3][MAXSHIFT] = {};
+  void release() {
+    if (w) hipFree(w);
+    if (off) hipFree(off);
+    w = nullptr;
+    off = nullptr;
+  }
+};
+
+void commit_structured_tables(nat_ctx* ctx, StructTables& t) {
+  if (ctx->ax_w) hipFree(ctx->ax_w);
+  if (ctx->ax_off) hipFree(ctx->ax_off);
+  ctx->ax_w = t.w;
+  ctx->ax_off = t.off;
+  ctx->ax_sym = t.sym;
+  std::memcpy(ctx->off_lo, t.off_lo, sizeof(t.off_lo));
+  std::memcpy(ctx->off_hi, t.off_hi, sizeof(t.off_hi));
+  t.w = nullptr;
+  t.off = nullptr;
+}
+
+// Structured 1-D tables on the GPU (FP64) for the context's current dt and vertices,
+// then host checks: offsets within one cell layer and no tensor-product weight the
+// reference would drop (|w| < 1e-10). *ok = false (and no tables) when a check fails.
+int build_structured_tables(nat_ctx* ctx, StructTables& out, bool* ok) {
   *ok = false;
   const int D = ctx->D, p = ctx->p;
   // shift id 0: component +s (departure at -dt s), 1: component -s
@@ -315,33 +362,30 @@ int setup_structured_tables(nat_ctx* ctx, bool* ok) {
     for (int i = 0; i <= ctx->desc.ncells[a]; i++)
       vaxis[a * (maxc + 1) + i] = vtx(ctx, a == 0 ? i : 0, a == 1 ? i : 0, a == 2 ? i : 0)[a];
   int N[3] = {ctx->desc.ncells[0], ctx->desc.ncells[1], D == 3 ? ctx->desc.ncells[2] : 1};
-  int *dN = nullptr, *dns = nullptr, *doff = nullptr;
-  double *dva = nullptr, *ddisp = nullptr, *dw = nullptr;
   const size_t nw = 3 * MAXSHIFT * MAXP * (MAXP + 1), no = 3 * MAXSHIFT * MAXP;
-  TRY(alloc(ctx, &dN, 3));
-  TRY(alloc(ctx, &dns, 3));
-  TRY(alloc(ctx, &doff, no));
-  TRY(alloc(ctx, &dva, vaxis.size()));
-  TRY(alloc(ctx, &ddisp, 3 * MAXSHIFT));
-  TRY(alloc(ctx, &dw, nw));
-  NAT_HIP(hipMemcpyAsync(dN, N, sizeof(N), hipMemcpyHostToDevice, ctx->stream));
-  NAT_HIP(hipMemcpyAsync(dns, nshift, sizeof(nshift), hipMemcpyHostToDevice, ctx->stream));
-  NAT_HIP(hipMemcpyAsync(dva, vaxis.data(), vaxis.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  NAT_HIP(hipMemcpyAsync(ddisp, disp, sizeof(disp), hipMemcpyHostToDevice, ctx->stream));
-  NAT_HIP(hipMemsetAsync(dw, 0, nw * sizeof(double), ctx->stream));
-  NAT_HIP(hipMemsetAsync(doff, 0, no * sizeof(int), ctx->stream));
-  TRY(nat_launch_assemble_axis(ctx, dN, dva, maxc, ddisp, dns, dw, doff));
   std::vector<double> w(nw);
   std::vector<int> off(no);
-  NAT_HIP(hipMemcpyAsync(w.data(), dw, nw * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  NAT_HIP(hipMemcpyAsync(off.data(), doff, no * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  NAT_HIP(hipStreamSynchronize(ctx->stream));
-  hipFree(dN);
-  hipFree(dns);
-  hipFree(doff);
-  hipFree(dva);
-  hipFree(ddisp);
-  hipFree(dw);
+  {
+    DevScratch tmp;
+    int *dN = nullptr, *dns = nullptr, *doff = nullptr;
+    double *dva = nullptr, *ddisp = nullptr, *dw = nullptr;
+    TRY(tmp.get(ctx, &dN, 3));
+    TRY(tmp.get(ctx, &dns, 3));
+    TRY(tmp.get(ctx, &doff, no));
+    TRY(tmp.get(ctx, &dva, vaxis.size()));
+    TRY(tmp.get(ctx, &ddisp, 3 * MAXSHIFT));
+    TRY(tmp.get(ctx, &dw, nw));
+    NAT_HIP(hipMemcpyAsync(dN, N, sizeof(N), hipMemcpyHostToDevice, ctx->stream));
+    NAT_HIP(hipMemcpyAsync(dns, nshift, sizeof(nshift), hipMemcpyHostToDevice, ctx->stream));
+    NAT_HIP(hipMemcpyAsync(dva, vaxis.data(), vaxis.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    NAT_HIP(hipMemcpyAsync(ddisp, disp, sizeof(disp), hipMemcpyHostToDevice, ctx->stream));
+    NAT_HIP(hipMemsetAsync(dw, 0, nw * sizeof(double), ctx->stream));
+    NAT_HIP(hipMemsetAsync(doff, 0, no * sizeof(int), ctx->stream));
+    TRY(nat_launch_assemble_axis(ctx, dN, dva, maxc, ddisp, dns, dw, doff));
+    NAT_HIP(hipMemcpyAsync(w.data(), dw, nw * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    NAT_HIP(hipMemcpyAsync(off.data(), doff, no * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    NAT_HIP(hipStreamSynchronize(ctx->stream));
+  }
   int st = 0;
   NAT_HIP(hipMemcpy(&st, ctx->d_status, sizeof(int), hipMemcpyDeviceToHost));
   if (st) {
@@ -349,6 +393,7 @@ int setup_structured_tables(nat_ctx* ctx, bool* ok) {
     return 0;
   }
   auto W = [&](int a, int s, int j, int m) { return w[((a * MAXSHIFT + s) * MAXP + j) * (MAXP + 1) + m]; };
+  StructTables t;
   for (int a = 0; a < D; a++)
     for (int s = 0; s < 2; s++) {
       int lo = 100, hi = -100;
@@ -358,8 +403,8 @@ int setup_structured_tables(nat_ctx* ctx, bool* ok) {
         hi = std::max(hi, o);
       }
       if (lo < -1 || hi > 1 || hi - lo > 1) return 0;
-      ctx->off_lo[a][s] = lo;
-      ctx->off_hi[a][s] = hi;
+      t.off_lo[a][s] = lo;
+      t.off_hi[a][s] = hi;
       // the kernel's fixed window pattern: for a positive component only the
       // vertex node (j = 0) reads the cell below; otherwise every node reads its own cell
       for (int j = 0; j < p; j++) {
@@ -412,17 +457,24 @@ int setup_structured_tables(nat_ctx* ctx, bool* ok) {
           if (std::fabs(W(a, 0, j, m) - wp) > tol) sym = false;
           if (std::fabs(W(a, 1, (p - j) % p, p - m) - wp) > tol) sym = false;
         }
-    ctx->ax_sym = sym;
+    t.sym = sym;
   }
   // FP64 weights for both storage precisions (the kernel interpolates in FP64)
   {
-    double* d;
-    TRY(alloc(ctx, &d, nw));
-    NAT_HIP(hipMemcpy(d, w.data(), nw * sizeof(double), hipMemcpyHostToDevice));
-    ctx->ax_w = d;
+    int rc = alloc(ctx, &t.w, nw);
+    if (!rc) rc = alloc(ctx, &t.off, no);
+    if (rc) {
+      t.release();
+      return rc;
+    }
+    hipError_t e = hipMemcpy(t.w, w.data(), nw * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.off, off.data(), no * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      t.release();
+      return fail(ctx, NAT_ERR_HIP, std::string("structured tables: ") + hipGetErrorString(e));
+    }
   }
-  TRY(alloc(ctx, &ctx->ax_off, no));
-  NAT_HIP(hipMemcpy(ctx->ax_off, off.data(), no * sizeof(int), hipMemcpyHostToDevice));
+  out = t;
   *ok = true;
   return 0;
 }
@@ -432,21 +484,6 @@ bool structured_instantiated(int D, int Q, int p) {
   if (D == 3) return (Q == 19 || Q == 27) && pq;
   return Q == 9 && pq;
 }
-
-// Device scratch freed when the set-up function returns, on every path.
-struct DevScratch {
-  std::vector<void*> bufs;
-  ~DevScratch() {
-    for (void* b : bufs)
-      if (b) hipFree(b);
-  }
-  template <typename T>
-  int get(nat_ctx* ctx, T** p, size_t count) {
-    int rc = alloc(ctx, p, count);
-    if (!rc) bufs.push_back(*p);
-    return rc;
-  }
-};
 
 int assemble_general(nat_ctx* ctx) {
   const int Qm1 = ctx->Q - 1;
@@ -551,6 +588,10 @@ void nat_destroy(nat_ctx* ctx) {
                   ctx->wall_flip, ctx->wall_bias};
   for (void* b : bufs)
     if (b) hipFree(b);
+  if (ctx->d_partials) hipFree(ctx->d_partials);
+  if (ctx->h_status) hipHostFree(ctx->h_status);
+  for (hipEvent_t e : ctx->ev)
+    if (e) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -620,6 +661,10 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
   TRY(alloc(ctx, &ctx->d_status, 1));
   TRY(alloc(ctx, &ctx->d_sums, 8));
   NAT_HIP(hipMemset(ctx->d_status, 0, sizeof(int)));
+  NAT_HIP(hipHostMalloc((void**)&ctx->h_status, sizeof(int), hipHostMallocDefault));
+  *ctx->h_status = 0;
+  NAT_HIP(hipEventCreate(&ctx->ev[0]));
+  NAT_HIP(hipEventCreate(&ctx->ev[1]));
   TRY(alloc(ctx, &ctx->d_vert, ctx->verts.size()));
   NAT_HIP(hipMemcpy(ctx->d_vert, ctx->verts.data(), ctx->verts.size() * sizeof(double), hipMemcpyHostToDevice));
   double gll[MAXP + 1], gw[MAXP + 1];
@@ -640,7 +685,9 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
     ctx->backend = want_struct ? NAT_BACKEND_STRUCTURED : NAT_BACKEND_GENERAL;
     if (ctx->backend == NAT_BACKEND_STRUCTURED) {
       bool ok = false;
-      TRY(setup_structured_tables(ctx, &ok));
+      StructTables t;
+      TRY(build_structured_tables(ctx, t, &ok));
+      if (ok) commit_structured_tables(ctx, t);
       if (!ok) {
         if (d.backend == NAT_BACKEND_STRUCTURED)
           return fail(ctx, NAT_ERR_UNSUPPORTED, "operator is not a separable one-cell-layer shift; use the general backend");
@@ -712,7 +759,8 @@ int nat_set_vertices(nat_ctx* ctx, const double* vertices) {
   const nat_desc& d = ctx->desc;
   const int nz = d.dim == 3 ? d.ncells[2] + 1 : 1;
   const size_t nv = (size_t)(d.ncells[0] + 1) * (d.ncells[1] + 1) * nz;
-  std::vector<double> old = ctx->verts;
+  const std::vector<double> old = ctx->verts;
+  const double old_hmin = ctx->h_min;
   for (size_t i = 0; i < nv; i++)
     for (int a = 0; a < d.dim; a++) ctx->verts[3 * i + a] = vertices[i * d.dim + a];
   if (d.cartesian && !is_axis_aligned(ctx)) {
@@ -723,21 +771,44 @@ int nat_set_vertices(nat_ctx* ctx, const double* vertices) {
     ctx->verts = old;
     return fail(ctx, NAT_ERR_UNSUPPORTED, "the structured backend needs a uniform Cartesian mesh");
   }
+  // Put the previous mesh back (host and device vertices, support points) after a failed
+  // rebuild, so that the operator in use stays consistent with the geometry.
+  auto restore = [&](int code, const std::string& msg) {
+    ctx->verts = old;
+    ctx->h_min = old_hmin;
+    if (hipMemcpy(ctx->d_vert, ctx->verts.data(), ctx->verts.size() * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        (d.scheme == NAT_SL && nat_launch_support_points(ctx) != 0))
+      ctx->assembled = false;
+    ctx->err = msg;
+    return code;
+  };
   NAT_HIP(hipMemcpy(ctx->d_vert, ctx->verts.data(), ctx->verts.size() * sizeof(double), hipMemcpyHostToDevice));
   ctx->h_min = min_vertex_distance(ctx);
   if (d.scheme != NAT_SL) return 0;  // SEDG evaluates the geometry from the vertices in every stage
-  TRY(nat_launch_support_points(ctx));
+  {
+    const int rc = nat_launch_support_points(ctx);
+    if (rc) return restore(rc, ctx->err);
+  }
   if (ctx->backend == NAT_BACKEND_STRUCTURED) {
-    if (ctx->ax_w) hipFree(ctx->ax_w);
-    if (ctx->ax_off) hipFree(ctx->ax_off);
-    ctx->ax_w = nullptr;
-    ctx->ax_off = nullptr;
     bool ok = false;
-    TRY(setup_structured_tables(ctx, &ok));
-    if (!ok) return fail(ctx, NAT_ERR_UNSUPPORTED, "moved mesh gives a non-separable operator for the structured backend");
+    StructTables t;
+    const int rc = build_structured_tables(ctx, t, &ok);
+    if (rc) return restore(rc, ctx->err);
+    if (!ok) return restore(NAT_ERR_UNSUPPORTED, "moved mesh gives a non-separable operator for the structured backend");
+    commit_structured_tables(ctx, t);
     return 0;
   }
-  return assemble_general(ctx);
+  const int rc = assemble_general(ctx);
+  if (rc) {
+    const std::string msg = ctx->err;
+    restore(rc, msg);
+    // the departure records were rewritten in place: rebuild them for the previous mesh
+    if (ctx->assembled && assemble_general(ctx) != 0) ctx->assembled = false;
+    ctx->err = msg;
+    return rc;
+  }
+  return 0;
 }
 
 int nat_set_stabilizer(nat_ctx* ctx, int on) {
@@ -748,22 +819,41 @@ int nat_set_stabilizer(nat_ctx* ctx, int on) {
 
 int nat_set_dt(nat_ctx* ctx, double dt) {
   nat_drop_graphs(ctx);
+  const double old_dt = ctx->dt, old_tau = ctx->tau;
   if (dt > 0) {
     ctx->dt = dt;
     ctx->tau = ctx->desc.viscosity / (ctx->dt * ctx->st.cs2) + 0.5;
   }
-  if (ctx->desc.scheme != NAT_SL) return nat_sedg_setup(ctx);
+  // a rejected time step leaves dt, tau and the operator as they were
+  auto restore = [&](int code, const std::string& msg) {
+    ctx->dt = old_dt;
+    ctx->tau = old_tau;
+    ctx->err = msg;
+    return code;
+  };
+  if (ctx->desc.scheme != NAT_SL) {
+    const int rc = nat_sedg_setup(ctx);
+    return rc ? restore(rc, ctx->err) : 0;
+  }
   if (ctx->backend == NAT_BACKEND_STRUCTURED) {
-    if (ctx->ax_w) hipFree(ctx->ax_w);
-    if (ctx->ax_off) hipFree(ctx->ax_off);
-    ctx->ax_w = nullptr;
-    ctx->ax_off = nullptr;
     bool ok = false;
-    TRY(setup_structured_tables(ctx, &ok));
-    if (!ok) return fail(ctx, NAT_ERR_UNSUPPORTED, "time step gives a non-separable operator for the structured backend");
+    StructTables t;
+    const int rc = build_structured_tables(ctx, t, &ok);
+    if (rc) return restore(rc, ctx->err);
+    if (!ok) return restore(NAT_ERR_UNSUPPORTED, "time step gives a non-separable operator for the structured backend");
+    commit_structured_tables(ctx, t);
     return 0;
   }
-  return assemble_general(ctx);
+  const int rc = assemble_general(ctx);
+  if (rc) {
+    const std::string msg = ctx->err;
+    restore(rc, msg);
+    // the departure records were rewritten in place: rebuild them for the previous dt
+    if (assemble_general(ctx) != 0) ctx->assembled = false;
+    ctx->err = msg;
+    return rc;
+  }
+  return 0;
 }
 
 // FP32 shift of the g populations per unit weight: rho0 (2 C_v - D), the g equilibrium at rest with T = 1
@@ -914,15 +1004,45 @@ int nat_collide(nat_ctx* ctx) {
   if (ctx->desc.compressible)
     TRY(nat_launch_collide_comp(ctx));
   else
-    TRY(nat_launch_collide(ctx));
+    TRY(nat_launch_collide(ctx, 0));
   ctx->collided = true;
+  return check_status(ctx);
+}
+
+int nat_collide_init(nat_ctx* ctx, const double* u) {
+  if (ctx->desc.compressible)
+    return fail(ctx, NAT_ERR_UNSUPPORTED, "the initialisation collision is built for incompressible contexts");
+  if (u) {
+    // the velocity field the collision reads: physical units, DoF order -> storage order
+    const int64_t n = ctx->n_dofs, ns = ctx->n_store;
+    const int D = ctx->D;
+    NAT_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->fp32) {
+      std::vector<float> h((size_t)ns * D);
+      for (int d = 0; d < D; d++)
+        for (int64_t i = 0; i < n; i++) h[(size_t)d * ns + store_index(ctx, i)] = (float)u[(size_t)d * n + i];
+      NAT_HIP(hipMemcpy(ctx->u, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+      std::vector<double> h((size_t)ns * D);
+      for (int d = 0; d < D; d++)
+        for (int64_t i = 0; i < n; i++) h[(size_t)d * ns + store_index(ctx, i)] = u[(size_t)d * n + i];
+      NAT_HIP(hipMemcpy(ctx->u, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  TRY(nat_launch_collide(ctx, 1));
+  ctx->collided = true;
+  return check_status(ctx);
+}
+
+int nat_stabilize(nat_ctx* ctx) {
+  TRY(nat_launch_stabilize(ctx));
   return check_status(ctx);
 }
 
 static int step_launch(nat_ctx* ctx, int nsteps, bool macro_last = true) {
   if (!ctx->assembled) return fail(ctx, NAT_ERR_STATE, "operator not assembled");
   for (int s = 0; s < nsteps; s++) {
-    const int wm = (macro_last && s == nsteps - 1) ? 1 : 0;
+    const int wm = (macro_last && s == nsteps - 1) ? STEP_MACRO : 0;
     if (ctx->desc.compressible) {
       TRY(nat_launch_sl_comp(ctx, wm));
     } else if (ctx->desc.scheme == NAT_SL) {
@@ -1001,26 +1121,27 @@ int nat_compressible_step(nat_ctx* ctx, int nsteps) {
 }
 
 int nat_step_timed(nat_ctx* ctx, int nsteps, double* elapsed_ms) {
-  hipEvent_t a, b;
-  NAT_HIP(hipEventCreate(&a));
-  NAT_HIP(hipEventCreate(&b));
-  NAT_HIP(hipEventRecord(a, ctx->stream));
-  int rc = step_enqueue(ctx, nsteps);
-  if (rc == 0) {
-    NAT_HIP(hipEventRecord(b, ctx->stream));
-    NAT_HIP(hipEventSynchronize(b));
-    float ms = 0;
-    NAT_HIP(hipEventElapsedTime(&ms, a, b));
-    *elapsed_ms = ms;
-  }
-  hipEventDestroy(a);
-  hipEventDestroy(b);
-  return rc ? rc : check_status(ctx);
+  NAT_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  TRY(step_enqueue(ctx, nsteps));
+  NAT_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  // check_status waits for the stream, which includes the second event
+  const int rc = check_status(ctx);
+  float ms = 0;
+  NAT_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  *elapsed_ms = ms;
+  return rc;
 }
 
 int nat_stream(nat_ctx* ctx) {
-  if (ctx->desc.scheme == NAT_SL) return fail(ctx, NAT_ERR_UNSUPPORTED, "SL streaming is fused with collision: use nat_step");
-  TRY(nat_sedg_step(ctx, 0, 0));
+  if (!ctx->assembled) return fail(ctx, NAT_ERR_STATE, "operator not assembled");
+  if (ctx->desc.compressible)
+    TRY(nat_launch_sl_comp(ctx, STEP_STREAM_ONLY));
+  else if (ctx->desc.scheme != NAT_SL)
+    TRY(nat_sedg_step(ctx, 0, 0));
+  else if (ctx->backend == NAT_BACKEND_STRUCTURED)
+    TRY(nat_launch_sl_struct(ctx, STEP_STREAM_ONLY));
+  else
+    TRY(nat_launch_sl_general(ctx, STEP_STREAM_ONLY));
   ctx->time += ctx->dt;
   return check_status(ctx);
 }
@@ -1073,6 +1194,12 @@ int nat_sums(nat_ctx* ctx, double* mass, double* momentum, double* ekin) {
     for (int d = 0; d < ctx->D; d++) momentum[d] = h[1 + d];
   if (ekin) *ekin = 0.5 * h[4];
   return 0;
+}
+
+int nat_struct_tile(nat_ctx* ctx, int* tile) {
+  if (ctx->desc.scheme != NAT_SL || ctx->backend != NAT_BACKEND_STRUCTURED || ctx->desc.compressible)
+    return fail(ctx, NAT_ERR_ARG, "nat_struct_tile: not a structured SL context");
+  return nat_launch_sl_struct_tile(ctx, tile);
 }
 
 int nat_sync(nat_ctx* ctx) {

@@ -82,6 +82,12 @@ struct MeshDev {
   double wall_u[6][3] = {};  // velocity-neq bounce-back wall velocity per box face (non-periodic axes)
 };
 
+// Mode bits of the fused step kernels' launch argument.
+enum {
+  STEP_MACRO = 1,       // write rho and u (the last step of a batch)
+  STEP_STREAM_ONLY = 2  // CFDSolver::stream without the collision: store the streamed values
+};
+
 struct GllTab {
   double x[MAXP + 1];
 };
@@ -135,7 +141,11 @@ struct nat_ctx {
   double* d_coords = nullptr;  // SL: FE_Q support points (n_dofs x 3)
   int* d_owner = nullptr;      // SL: owning cell per FE_Q DoF
   int* d_status = nullptr;
+  int* h_status = nullptr;     // pinned host copy of the status word (one D2H copy per call, no staging)
   double* d_sums = nullptr;
+  double* d_partials = nullptr;  // per-block partial conservation sums (two-pass reduction)
+  int64_t n_partials = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // timing events of nat_step_timed, created once
 
   // SL general backend: departure records per (alpha-1, node)
   int* dep_cell = nullptr;

@@ -135,3 +135,24 @@ def couette2d_velocity(y, t, U, L=1.0, viscosity=1.0):
         if e < 1e-20:
             break
     return s
+
+
+def serving_mix(n, seed, full=True):
+    """Seeded serving workload: 2D or 3D Taylor-Green vortex, or a shock tube on a 2D strip;
+    refinement 3-5, p in {2, 3, 4, 6}, D2Q9 (2D) or D3Q19 / D3Q27 (3D), Re log-uniform in
+    [10, 1600], Ma uniform in [0.05, 0.2], step count log-uniform in [100, 5000].
+    full=False caps refinement at 4 (3 for 3D at p = 6) and steps at 1000."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        prob = ["TGV2D", "TGV3D", "SHOCK_TUBE"][rng.integers(3)]
+        ref = int(rng.integers(3, 6 if full else 5))
+        p = int([2, 3, 4, 6][rng.integers(4)])
+        st = "D2Q9" if prob != "TGV3D" else ["D3Q19", "D3Q27"][rng.integers(2)]
+        if prob == "TGV3D" and not full:
+            ref = min(ref, 3 if p == 6 else 4)
+        re_ = float(math.exp(rng.uniform(math.log(10), math.log(1600))))
+        ma = float(rng.uniform(0.05, 0.2))
+        steps = int(math.exp(rng.uniform(math.log(100), math.log(5000 if full else 1000))))
+        out.append(dict(problem=prob, stencil=st, refinement=ref, order=p, reynolds=re_, mach=ma, nsteps=steps))
+    return out

@@ -76,6 +76,8 @@ def lib():
         L.orc_op_src.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int8)]
         L.orc_op_bvec.restype = ctypes.c_int
         L.orc_op_bvec.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_collide_init.restype = ctypes.c_int
+        L.orc_collide_init.argtypes = [ctypes.c_void_p, _dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -245,6 +247,14 @@ class Oracle:
 
     def collide(self):
         self._check(lib().orc_collide(self.h))
+
+    def collide_init(self, u=None):
+        """Initialisation collision: the stored velocity (u if given) drives the relaxation."""
+        up = None
+        if u is not None:
+            u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.D, self.n)
+            up = _p(u)
+        self._check(lib().orc_collide_init(self.h, up))
 
     def stream(self):
         self._check(lib().orc_stream(self.h))

@@ -223,6 +223,23 @@ int orc_collide(orc_solver* s) {
   return e;
 }
 
+// selectCollision(..., init = true) (CollisionOperator.h:79-91): the stored velocity
+// (replaced by u first when u is not null) drives the relaxation; incompressible only.
+int orc_collide_init(orc_solver* s, const double* u) {
+  if (s->compressible) {
+    s->err = "initialisation collision: incompressible only";
+    return -1;
+  }
+  const int Q = s->st.Q, D = s->st.D;
+  if (u) std::memcpy(s->u.data(), u, sizeof(double) * s->n * D);
+  auto fp = s->fptr(s->f, Q);
+  std::vector<double*> up(D);
+  for (int d = 0; d < D; d++) up[d] = s->u.data() + (size_t)d * s->n;
+  const int e = collide_bgk(s->st, s->cp, s->n, fp.data(), s->rho.data(), up.data(), true);
+  if (e) s->err = "Densities too small (< 1e-10) for collisions. Decrease time step size.";
+  return e;
+}
+
 // One SL / SEDG streaming of a population vector. For SL, with_boundary adds the
 // velocity-neq bounce-back values (applyBoundaryConditions after the product,
 // CFDSolver.cpp:669-721); the g populations of the compressible solver get the

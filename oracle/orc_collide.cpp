@@ -190,7 +190,7 @@ void feq_quartic(const GenData& g, const HTensors& h, double rho, const double* 
 }  // namespace
 
 int collide_bgk(const Stencil& st, const CollisionParams& cp, int64_t n, double* const* f, double* rho,
-                double* const* u) {
+                double* const* u, bool init) {
   const GenData g = make_gendata(st, cp);
   int err = 0;
 #pragma omp parallel for schedule(static) reduction(| : err)
@@ -202,7 +202,10 @@ int collide_bgk(const Stencil& st, const CollisionParams& cp, int64_t n, double*
     if (density < 1e-10) err |= 1;
     rho[ii] = density;
     velocity(g, fl, density, vel);
-    for (int j = 0; j < g.D; ++j) u[j][ii] = vel[j] * g.scaling;
+    if (!init)
+      for (int j = 0; j < g.D; ++j) u[j][ii] = vel[j] * g.scaling;
+    else
+      for (int j = 0; j < g.D; ++j) vel[j] = u[j][ii] / g.scaling;
     feq_bgk(g, density, vel, feq);
     for (int p = 0; p < g.Q; ++p) fl[p] -= 1. / g.tau * (fl[p] - feq[p]);
     for (int p = 0; p < g.Q; ++p) f[p][ii] = fl[p];

@@ -27,8 +27,10 @@ struct CollisionParams {
 
 // f: Q arrays of n; rho: n; u: D arrays of n (written as u*scaling).
 // Returns 0, or 1 if a density fell below 1e-10 (the reference throws).
+// init: the initialisation collision (CollisionOperator.h:79-91), u is read (u / scaling
+// replaces the computed velocity) and not written.
 int collide_bgk(const Stencil& st, const CollisionParams& cp, int64_t n, double* const* f, double* rho,
-                double* const* u);
+                double* const* u, bool init = false);
 
 // Compressible BGK with quartic equilibrium on f and g; writes rho, u*scaling, T, mask.
 int collide_compressible(const Stencil& st, const CollisionParams& cp, int64_t n, double* const* f, double* const* g,
