@@ -191,6 +191,15 @@ int nat_get_knudsen(nat_ctx* ctx, double* kn);
  * plus total momentum, reduced on the device in FP64 from the last collision's fields. */
 int nat_sums(nat_ctx* ctx, double* mass, double* momentum /* dim */, double* ekin);
 
+/* nat_sums without any device allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). The partial sums go to a fixed per-GPU scratch of NAT_SUMS_SCRATCH_BLOCKS x 5
+ * doubles (2.5 MiB) held until the process ends, instead of the context's lazily allocated
+ * buffer; meshes with more than 16,777,216 (cell, node) pairs return NAT_ERR_ARG. The result
+ * is bit-identical to nat_sums. Calls that overlap on one GPU (two streams or threads at
+ * once) share the scratch and race: serialise them. */
+int nat_sums_noalloc(nat_ctx* ctx, double* mass, double* momentum /* dim */, double* ekin);
+
 int nat_sync(nat_ctx* ctx);
 
 /* Launch shape of the structured SL step kernel, no counterpart in the reference: the node

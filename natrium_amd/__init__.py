@@ -56,7 +56,7 @@ EXPORTS = [
     "nat_upload_f", "nat_download_f", "nat_upload_g", "nat_download_g", "nat_collide", "nat_collide_init",
     "nat_stream", "nat_stabilize",
     "nat_step", "nat_sl_assemble", "nat_sl_bgk_step", "nat_sedg_rk5_step", "nat_compressible_step",
-    "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sync",
+    "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sums_noalloc", "nat_sync",
     "nat_stats", "nat_write_stats", "nat_write_vtu", "nat_checkpoint_write", "nat_checkpoint_read",
     "nat_struct_tile",
 ]
@@ -127,8 +127,9 @@ def lib():
     L.nat_get_macro.argtypes = [vp, _dp, _dp, _dp]
     L.nat_get_knudsen.restype = ctypes.c_int
     L.nat_get_knudsen.argtypes = [vp, _dp]
-    L.nat_sums.restype = ctypes.c_int
-    L.nat_sums.argtypes = [vp, _dp, _dp, _dp]
+    for nm in ("nat_sums", "nat_sums_noalloc"):
+        getattr(L, nm).restype = ctypes.c_int
+        getattr(L, nm).argtypes = [vp, _dp, _dp, _dp]
     L.nat_struct_tile.restype = ctypes.c_int
     L.nat_struct_tile.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     _lib = L
@@ -329,11 +330,14 @@ class Solver:
         self._check(lib().nat_get_knudsen(self._h, _ptr(kn)))
         return kn
 
-    def sums(self):
+    def sums(self, noalloc=False):
+        """(mass, momentum, E_kin) of the last collision's fields. noalloc: through
+        nat_sums_noalloc (fixed per-GPU scratch, no device allocation)."""
         m = np.zeros(1)
         mom = np.zeros(3)
         e = np.zeros(1)
-        self._check(lib().nat_sums(self._h, _ptr(m), _ptr(mom), _ptr(e)))
+        fn = lib().nat_sums_noalloc if noalloc else lib().nat_sums
+        self._check(fn(self._h, _ptr(m), _ptr(mom), _ptr(e)))
         return float(m[0]), mom[:self.D].copy(), float(e[0])
 
     def stats(self, iteration):

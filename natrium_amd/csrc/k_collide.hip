@@ -1,6 +1,9 @@
 // Collision-only pass (CFDSolver::collide, CFDSolver.cpp:807-843) and FP64
 // conservation sums (PhysicalProperties::mass / kineticEnergy,
 // PhysicalProperties.cpp:29-130 and 270-310).
+#include <mutex>
+#include <string>
+
 #include "nat_collide.hpp"
 #include "nat_geom.hpp"
 
@@ -233,7 +236,8 @@ int nat_launch_stabilize(nat_ctx* ctx) {
   return 0;
 }
 
-int nat_launch_sums(nat_ctx* ctx, int layout) {
+// Quadrature-sum arguments of a context and the number of pass-1 blocks they need.
+static nat::SumArgs sum_args(const nat_ctx* ctx, int layout, int64_t* grid) {
   using namespace nat;
   SumArgs s;
   s.m.D = ctx->D;
@@ -250,7 +254,29 @@ int nat_launch_sums(nat_ctx* ctx, int layout) {
   gll_points(ctx->p, s.gll, s.gllw);
   const int np = ctx->p + 1;
   const int64_t total = (int64_t)s.m.N[0] * s.m.N[1] * s.m.N[2] * (ctx->D == 3 ? np * np * np : np * np);
-  const int64_t grid = (total + SUM_BS - 1) / SUM_BS;
+  *grid = (total + SUM_BS - 1) / SUM_BS;
+  return s;
+}
+
+// Both passes on the context's stream; partial holds 5 doubles per pass-1 block.
+static int launch_sums_into(nat_ctx* ctx, const nat::SumArgs& s, int64_t grid, double* partial) {
+  using namespace nat;
+  if (ctx->fp32)
+    hipLaunchKernelGGL((k_sums<float>), dim3((unsigned)grid), dim3(SUM_BS), 0, ctx->stream, s,
+                       (const float*)ctx->rho, (const float*)ctx->u, partial);
+  else
+    hipLaunchKernelGGL((k_sums<double>), dim3((unsigned)grid), dim3(SUM_BS), 0, ctx->stream, s,
+                       (const double*)ctx->rho, (const double*)ctx->u, partial);
+  NAT_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(SUM_BS), 0, ctx->stream, (const double*)partial, grid,
+                     ctx->d_sums);
+  NAT_HIP(hipGetLastError());
+  return 0;
+}
+
+int nat_launch_sums(nat_ctx* ctx, int layout) {
+  int64_t grid = 0;
+  const nat::SumArgs s = sum_args(ctx, layout, &grid);
   if (grid > ctx->n_partials) {
     if (ctx->d_partials) hipFree(ctx->d_partials);
     ctx->d_partials = nullptr;
@@ -258,15 +284,59 @@ int nat_launch_sums(nat_ctx* ctx, int layout) {
     NAT_HIP(hipMalloc((void**)&ctx->d_partials, (size_t)grid * 5 * sizeof(double)));
     ctx->n_partials = grid;
   }
-  if (ctx->fp32)
-    hipLaunchKernelGGL((k_sums<float>), dim3((unsigned)grid), dim3(SUM_BS), 0, ctx->stream, s,
-                       (const float*)ctx->rho, (const float*)ctx->u, ctx->d_partials);
-  else
-    hipLaunchKernelGGL((k_sums<double>), dim3((unsigned)grid), dim3(SUM_BS), 0, ctx->stream, s,
-                       (const double*)ctx->rho, (const double*)ctx->u, ctx->d_partials);
-  NAT_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(SUM_BS), 0, ctx->stream, (const double*)ctx->d_partials, grid,
-                     ctx->d_sums);
-  NAT_HIP(hipGetLastError());
+  return launch_sums_into(ctx, s, grid, ctx->d_partials);
+}
+
+// ---------------------------------------------------------------------------
+// Sums without allocation: the pass-1 partial sums go to a fixed device array instead of
+// the context's lazily allocated buffer. Each GPU holds its own copy of the array at its
+// own address, looked up once per device with that device current (nat_sums_scratch_setup,
+// called from nat_create); host code uses only the looked-up address, never the array's
+// name. Every pass-1 block overwrites its own 5 entries, so nothing has to start at zero;
+// calls that overlap on one GPU (two streams at once) share the array and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[NAT_SUMS_SCRATCH_BLOCKS * 5];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int nat_sums_scratch_setup(nat_ctx* ctx) {
+  int dev = 0;
+  NAT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    ctx->err = "sums scratch: device index beyond the supported range";
+    return NAT_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    NAT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  ctx->device = dev;
   return 0;
+}
+
+int nat_launch_sums_noalloc(nat_ctx* ctx, int layout) {
+  int64_t grid = 0;
+  const nat::SumArgs s = sum_args(ctx, layout, &grid);
+  if (grid > NAT_SUMS_SCRATCH_BLOCKS) {
+    ctx->err = "nat_sums_noalloc: " + std::to_string(grid * nat::SUM_BS) +
+               " (cell, node) pairs exceed the fixed scratch of " +
+               std::to_string((int64_t)NAT_SUMS_SCRATCH_BLOCKS * nat::SUM_BS) + "; use nat_sums";
+    return NAT_ERR_ARG;
+  }
+  double* part = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (ctx->device >= 0 && ctx->device < kMaxDevices) part = g_scratch_ptr[ctx->device];
+  }
+  if (!part) {
+    ctx->err = "nat_sums_noalloc: the device's scratch address was not looked up at context creation";
+    return NAT_ERR_STATE;
+  }
+  return launch_sums_into(ctx, s, grid, part);
 }

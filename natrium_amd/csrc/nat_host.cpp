@@ -25,6 +25,8 @@ int nat_launch_sl_general(nat_ctx* ctx, int write_macro);
 int nat_launch_collide(nat_ctx* ctx, int init);
 int nat_launch_stabilize(nat_ctx* ctx);
 int nat_launch_sums(nat_ctx* ctx, int layout);
+int nat_launch_sums_noalloc(nat_ctx* ctx, int layout);
+int nat_sums_scratch_setup(nat_ctx* ctx);
 int nat_sedg_setup(nat_ctx* ctx);
 int nat_sedg_step(nat_ctx* ctx, int collide_after, int write_macro);
 int nat_comp_setup(nat_ctx* ctx);
@@ -665,6 +667,7 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
   *ctx->h_status = 0;
   NAT_HIP(hipEventCreate(&ctx->ev[0]));
   NAT_HIP(hipEventCreate(&ctx->ev[1]));
+  TRY(nat_sums_scratch_setup(ctx));  // this device's address of the fixed sums scratch
   TRY(alloc(ctx, &ctx->d_vert, ctx->verts.size()));
   NAT_HIP(hipMemcpy(ctx->d_vert, ctx->verts.data(), ctx->verts.size() * sizeof(double), hipMemcpyHostToDevice));
   double gll[MAXP + 1], gw[MAXP + 1];
@@ -1183,9 +1186,7 @@ int nat_get_macro(nat_ctx* ctx, double* rho, double* u, double* T) {
   return 0;
 }
 
-int nat_sums(nat_ctx* ctx, double* mass, double* momentum, double* ekin) {
-  const int layout = ctx->desc.scheme != NAT_SL ? 2 : (ctx->backend == NAT_BACKEND_STRUCTURED ? 1 : 0);
-  TRY(nat_launch_sums(ctx, layout));
+static int sums_result(nat_ctx* ctx, double* mass, double* momentum, double* ekin) {
   double h[5];
   NAT_HIP(hipMemcpyAsync(h, ctx->d_sums, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   NAT_HIP(hipStreamSynchronize(ctx->stream));
@@ -1194,6 +1195,20 @@ int nat_sums(nat_ctx* ctx, double* mass, double* momentum, double* ekin) {
     for (int d = 0; d < ctx->D; d++) momentum[d] = h[1 + d];
   if (ekin) *ekin = 0.5 * h[4];
   return 0;
+}
+
+static int sums_layout(const nat_ctx* ctx) {
+  return ctx->desc.scheme != NAT_SL ? 2 : (ctx->backend == NAT_BACKEND_STRUCTURED ? 1 : 0);
+}
+
+int nat_sums(nat_ctx* ctx, double* mass, double* momentum, double* ekin) {
+  TRY(nat_launch_sums(ctx, sums_layout(ctx)));
+  return sums_result(ctx, mass, momentum, ekin);
+}
+
+int nat_sums_noalloc(nat_ctx* ctx, double* mass, double* momentum, double* ekin) {
+  TRY(nat_launch_sums_noalloc(ctx, sums_layout(ctx)));
+  return sums_result(ctx, mass, momentum, ekin);
 }
 
 int nat_struct_tile(nat_ctx* ctx, int* tile) {

@@ -108,6 +108,10 @@ struct CollideArgs {
 
 }  // namespace nat
 
+// Pass-1 blocks (256 (cell, node) pairs each) that the fixed scratch of nat_sums_noalloc
+// holds: 65536 x 5 doubles = 2.5 MiB per GPU, up to 16,777,216 pairs.
+#define NAT_SUMS_SCRATCH_BLOCKS 65536
+
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
@@ -127,6 +131,7 @@ struct nat_ctx {
   bool collided = false;
   int backend = NAT_BACKEND_GENERAL;
   hipStream_t stream = nullptr;
+  int device = -1;  // HIP device current at nat_create (the context's allocations live there)
 
   // device state
   void* f[2] = {nullptr, nullptr};
