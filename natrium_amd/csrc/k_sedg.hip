@@ -130,6 +130,11 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
 
   T vnext = active ? s.fin[1 * s.stride + node] : T(0);
   static_for<1, Q>([&](auto A) {
+    // The stage arithmetic states its fused multiply-adds (fma) and forbids the compiler
+    // any other contraction: a contraction left to the compiler's choice can differ
+    // between the stream-only and the collide-after instantiations of this kernel, and
+    // the two must give the same populations and fields bit for bit.
+#pragma clang fp contract(off)
     constexpr int al = decltype(A)::value;
     constexpr int e0 = L::e[al][0], e1 = L::e[al][1], e2 = (D == 3) ? L::e[al][2] : 0;
     // per-direction addresses are formed here, not hoisted into long-lived registers
@@ -175,8 +180,8 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
       const int base = tid - mi[b] * lstr[b];
       double der = 0;
 #pragma unroll
-      for (int j = 0; j < NP; j++) der += dl[mi[b]][j] * buf[al & 1][base + j * lstr[b]];
-      lf -= G[b] * der;
+      for (int j = 0; j < NP; j++) der = fma(dl[mi[b]][j], buf[al & 1][base + j * lstr[b]], der);
+      lf = fma(-G[b], der, lf);
     }
     if constexpr (!early) load_faces();
 #pragma unroll
@@ -189,12 +194,11 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
         }
       } else if (c < 0.0) {
         const double fn = (double)fnb[a];
-        lf += c * (v - fn);
+        lf = fma(c, v - fn, lf);
       }
     }
-    double af = lf;
-    af *= dt;
-    double d = s.first ? af : (double)dfo * ra + af;
+    const double af = lf * dt;
+    double d = s.first ? af : fma((double)dfo, ra, af);
     d *= rb;
     const double fnew = v + d;
     d *= s.binv;
