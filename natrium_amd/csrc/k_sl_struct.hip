@@ -348,6 +348,7 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
 
   // ---- one 1-D pass of direction al (slot b, output parity par)
   auto run_pass = [&](auto ALc, auto Kc, auto Bc, auto PARc) {
+#pragma clang fp contract(off)  // the one fma below is explicit: all instantiations round alike
     constexpr int al = decltype(ALc)::value, kp = decltype(Kc)::value;
     constexpr int b = decltype(Bc)::value, par = decltype(PARc)::value;
     constexpr int ps = decltype(PARc)::grp % PF;  // register slot set of this group
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE,
               w = (G.C > 0) ? W0[(kk + 1) % P][m] : W0[(P - kk) % P][P - m];
             else
               w = wt[((kk + (G.C > 0 ? 1 : 0)) % P) * (MAXP + 1) + m];
-            acc[kk] = (m == 0) ? w * v[0] : acc[kk] + w * v[m];
+            acc[kk] = (m == 0) ? w * v[0] : fma(w, v[m], acc[kk]);
           }
         }
         // output kk is the node sc*P + kk (+1-P for a positive component) of the tile

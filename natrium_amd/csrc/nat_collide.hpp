@@ -61,6 +61,7 @@ __device__ __forceinline__ StabPi<Q> stab_moments(const double (&f)[Q]) {
 template <int Q>
 __device__ __forceinline__ StabPi<Q> stab_neq(StabPi<Q> m, double drho, double rho, const double (&u)[3],
                                               double om) {
+#pragma clang fp contract(off)
   const double k = (1.0 - om) * 4.5;
   const double c = drho / 3.0;
   m.xx = k * (m.xx - rho * u[0] * u[0] - c);
@@ -74,6 +75,7 @@ __device__ __forceinline__ StabPi<Q> stab_neq(StabPi<Q> m, double drho, double r
 
 template <int Q>
 __device__ __forceinline__ double stab_term(const StabPi<Q>& m, int i) {
+#pragma clang fp contract(off)
   using L = Lattice<Q>;
   const int ex = L::e[i][0], ey = L::e[i][1], ez = L::e[i][2];
   double v = 0;
@@ -113,6 +115,9 @@ constexpr int opposite(int i) {
 template <int Q, typename T, bool INIT = false>
 __device__ __forceinline__ void bgk_collide_store_pairs(const T (&f)[Q], const CollideArgs<T>& ca, int64_t node,
                                                         bool write_macro, T* __restrict__ fout, int64_t stride) {
+  // Fused multiply-adds are written out and no other contraction is allowed: every kernel
+  // that inlines this function rounds the same way, whatever the compiler decides elsewhere.
+#pragma clang fp contract(off)
   using L = Lattice<Q>;
   constexpr int D = L::D;
   double fd[Q];
@@ -144,8 +149,8 @@ __device__ __forceinline__ void bgk_collide_store_pairs(const T (&f)[Q], const C
   }
   double uu = 0;
 #pragma unroll
-  for (int d = 0; d < D; d++) uu += u[d] * u[d];
-  const double base = drho - rho * (1.5 * uu);  // even part without the ue^2 term
+  for (int d = 0; d < D; d++) uu = fma(u[d], u[d], uu);
+  const double base = fma(-rho, 1.5 * uu, drho);  // even part without the ue^2 term
   const double hr = 0.5 * rho;
   const double om = ca.omega64;
   const uint32_t nb = (uint32_t)node * (uint32_t)sizeof(T);
@@ -170,7 +175,7 @@ __device__ __forceinline__ void bgk_collide_store_pairs(const T (&f)[Q], const C
     // stabilized: f_eq + (1 - omega) P f_neq; the correction is even in e
     double uu3[3] = {u[0], u[1], D == 3 ? u[2] : 0.0};
     const StabPi<Q> pn = stab_neq<Q>(stab_moments<Q>(fd), drho, rho, uu3, om);
-    store(0, L::w[0] * base + stab_term<Q>(pn, 0));
+    store(0, fma(L::w[0], base, stab_term<Q>(pn, 0)));
     static_for<1, Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
       constexpr int o = opposite<Q>(i);
@@ -181,15 +186,15 @@ __device__ __forceinline__ void bgk_collide_store_pairs(const T (&f)[Q], const C
           if (L::e[i][d] > 0) ue += u3[d];
           if (L::e[i][d] < 0) ue -= u3[d];
         }
-        const double E = base + hr * (ue * ue);
+        const double E = fma(hr, ue * ue, base);
         const double O = rho * ue;
-        const double wE = L::w[i] * E + stab_term<Q>(pn, i), wO = L::w[i] * O;
+        const double wE = fma(L::w[i], E, stab_term<Q>(pn, i)), wO = L::w[i] * O;
         store(i, wE + wO);
         store(o, wE - wO);
       }
     });
   } else {
-    store(0, fd[0] - om * (fd[0] - L::w[0] * base));
+    store(0, fma(-om, fd[0] - L::w[0] * base, fd[0]));
     static_for<1, Q>([&](auto I) {
       constexpr int i = decltype(I)::value;
       constexpr int o = opposite<Q>(i);
@@ -200,11 +205,11 @@ __device__ __forceinline__ void bgk_collide_store_pairs(const T (&f)[Q], const C
           if (L::e[i][d] > 0) ue += u3[d];
           if (L::e[i][d] < 0) ue -= u3[d];
         }
-        const double E = base + hr * (ue * ue);
+        const double E = fma(hr, ue * ue, base);
         const double O = rho * ue;
         const double wE = L::w[i] * E, wO = L::w[i] * O;
-        store(i, fd[i] - om * (fd[i] - (wE + wO)));
-        store(o, fd[o] - om * (fd[o] - (wE - wO)));
+        store(i, fma(-om, fd[i] - (wE + wO), fd[i]));
+        store(o, fma(-om, fd[o] - (wE - wO), fd[o]));
       }
     });
   }
