@@ -6,6 +6,11 @@
  * no C++ type crosses this boundary. Populations and operators stay in device
  * memory between calls; each context owns one HIP stream and calls on different
  * contexts may run from different threads. Calls on one context must not overlap.
+ * nat_step captures a HIP graph of its steps the first time it runs more than 32 steps
+ * from a given ping-pong state; the HIP runtime fails a capture, and the other call, when
+ * another thread meanwhile issues work that synchronises with the legacy default stream
+ * (creating or initialising a context, copies), so threads that do that concurrently
+ * serialise it against nat_step calls of that length (the request server does).
  *
  * Array conventions (host pointers unless the name says device):
  *   DoF numbering   SL:   lattice nodes (N_a p + 1 per axis), x fastest; nodes on
@@ -207,6 +212,28 @@ int nat_sync(nat_ctx* ctx);
  * owns on this mesh. */
 int nat_struct_tile(nat_ctx* ctx, int* tile /* 3 */);
 
+/* Packed stepping, no counterpart in the reference (which runs one solve per process): for
+ * callers that advance many small solves, e.g. a request server. nat_step_packed runs nsteps
+ * fused stream + collide steps of each of the n contexts (1 <= n <= 16) as one kernel launch
+ * per step that advances all of them, on the first context's stream (after the pending work
+ * of every member; their later work waits for it). Members must be distinct, on one GPU, and
+ * share a packing key: structured SL contexts (uniform periodic boxes) at p = 2-4 with the same
+ * dimension, stencil, order, storage precision and tile. nat_pack_key returns that key, or
+ * NAT_ERR_UNSUPPORTED for a context that cannot be packed. Every member's populations and
+ * fields are bit-identical to those of nat_step(member, nsteps); an error of member i is
+ * reported on the first context with the prefix "context i". */
+int nat_pack_key(nat_ctx* ctx, int64_t* key);
+int nat_step_packed(nat_ctx* const* ctxs, int n, int nsteps);
+
+/* Tile autotuner, no counterpart in the reference: times every tile shape the structured SL
+ * step kernel is built with for this (stencil, order) on the context's own mesh, nsteps steps
+ * each (nsteps <= 0: 10), and keeps the fastest for this context and, process-wide, for later
+ * contexts of the same kernel instantiation and mesh size (which then skip the timing). The
+ * timing launches write only the idle population buffer: populations, fields and time are
+ * unchanged, and results do not depend on the tile. Structured SL contexts only; tile receives
+ * the chosen (x, y, z) extents. */
+int nat_autotune(nat_ctx* ctx, int nsteps, int* tile /* 3, may be NULL */);
+
 /* ---- Output and restart (host side, from fields downloaded through this API) ---- */
 
 /* One row of the reference's results table: SolverStats::update/printNewLine
@@ -215,9 +242,11 @@ int nat_struct_tile(nat_ctx* ctx, int* tile /* 3 */);
  * int |grad u|_F^2 in 3D on the GLL quadrature (enstrophy_sq the integral of its square),
  * entropy is -sum_cells (sum_q JxW_q) sum_i sum_j f_j ln(f_j / w_j), pressures are
  * (rho - 1) c_s^2, mean(ux) the volume-weighted mean of the cell DoF averages. Residuals are
- * formed at iterations divisible by 10 against the fields of the previous such call, once 100
- * iterations have passed since the first (SolverStats::calculateResiduals); otherwise the
- * last values are repeated. */
+ * formed at iterations divisible by 100 against the fields of the previous such call, once 100
+ * iterations have passed since the first (SolverStats::calculateResiduals, which the
+ * reference's stop check calls every 100 iterations, CFDSolver.cpp:927-931); otherwise, and on
+ * a second call at the same iteration, the last values are repeated (1.0 before the first
+ * residual, CFDSolver.cpp:398-399). */
 typedef struct nat_stats_t {
   int64_t iteration;
   double time, max_u, mass, kin_e, enstrophy, entropy, max_p, min_p, res_rho, res_u, mean_ux, enstrophy_sq;
@@ -246,10 +275,12 @@ int nat_checkpoint_read(nat_ctx* ctx, const char* dir, int64_t iteration, int64_
  * Request server. No counterpart in the reference (one solve per process):
  * continuous batching of independent solves on one GPU. Requests are admitted
  * in arrival order while their device footprint fits the HBM budget and an
- * active slot is free; every active request runs on its own stream in chunks of
- * HIP-graph-captured fused steps, so requests overlap on the device and a
- * retired request's memory admits the next one. A request's results do not
- * depend on the other requests in flight. All functions are thread-safe.
+ * active slot is free, and built on setup threads. Active requests advance in
+ * chunks of fused steps: alone on their own stream as replays of a captured HIP
+ * graph, or, when several idle requests share a packing key, together in packed
+ * launches (one launch per step for all of them), so requests overlap on the
+ * device and a retired request's memory admits the next one. A request's results
+ * do not depend on the other requests in flight. All functions are thread-safe.
  * ------------------------------------------------------------------------ */
 enum { NAT_PROBLEM_TGV2D = 0, NAT_PROBLEM_TGV3D = 1, NAT_PROBLEM_SHOCK_TUBE = 2 };
 
@@ -274,6 +305,7 @@ typedef struct nat_result {
   double latency_ms;       /* submit -> retired */
   double mass, ekin;       /* FP64 conservation sums after the last step */
   int64_t footprint_bytes; /* device bytes reserved for the request */
+  char error[256];         /* message of a failed request, empty otherwise */
 } nat_result;
 
 typedef struct nat_server_opts {
@@ -281,12 +313,18 @@ typedef struct nat_server_opts {
   int max_active;           /* concurrently resident requests (<= 0: 8) */
   int chunk_steps;          /* steps per scheduling chunk (<= 0: 50) */
   int use_graphs;           /* 1: replay one captured HIP graph per chunk */
+  int pack;                 /* 1: idle requests with equal nat_pack_key advance together in
+                               packed launches (nat_step_packed), up to 16 per launch */
+  int autotune;             /* 1: pick each structured request's tile with nat_autotune */
+  int setup_threads;        /* threads building admitted requests (mesh, operator, initial
+                               state) beside the stepping (<= 0: 4) */
 } nat_server_opts;
 
 typedef struct nat_server_stats_t {
   int64_t queued, active, used_bytes;
-  int64_t node_updates; /* DoF updates of completed requests */
-  double busy_ms;       /* first admission -> last retirement */
+  int64_t node_updates;  /* DoF updates of completed requests */
+  double busy_ms;        /* first admission -> last retirement */
+  int64_t packed_chunks; /* chunks issued as packed launches of several requests */
 } nat_server_stats_t;
 
 typedef struct nat_server nat_server;

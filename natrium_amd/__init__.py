@@ -58,7 +58,7 @@ EXPORTS = [
     "nat_step", "nat_sl_assemble", "nat_sl_bgk_step", "nat_sedg_rk5_step", "nat_compressible_step",
     "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sums_noalloc", "nat_sync",
     "nat_stats", "nat_write_stats", "nat_write_vtu", "nat_checkpoint_write", "nat_checkpoint_read",
-    "nat_struct_tile",
+    "nat_struct_tile", "nat_pack_key", "nat_step_packed", "nat_autotune",
 ]
 
 _lib = None
@@ -132,6 +132,12 @@ def lib():
         getattr(L, nm).argtypes = [vp, _dp, _dp, _dp]
     L.nat_struct_tile.restype = ctypes.c_int
     L.nat_struct_tile.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.nat_pack_key.restype = ctypes.c_int
+    L.nat_pack_key.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.nat_step_packed.restype = ctypes.c_int
+    L.nat_step_packed.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int]
+    L.nat_autotune.restype = ctypes.c_int
+    L.nat_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     _lib = L
     return L
 
@@ -370,6 +376,32 @@ class Solver:
         self._check(lib().nat_struct_tile(self._h, t))
         return tuple(t)
 
+    def pack_key(self):
+        """Packing key (nat_pack_key): solves with equal keys can share packed launches."""
+        k = ctypes.c_int64(0)
+        self._check(lib().nat_pack_key(self._h, ctypes.byref(k)))
+        return k.value
+
+    def autotune(self, nsteps=10):
+        """Time the structured kernel's tile shapes on this mesh and keep the fastest
+        (nat_autotune); returns the chosen tile."""
+        t = (ctypes.c_int * 3)()
+        self._check(lib().nat_autotune(self._h, int(nsteps), t))
+        return tuple(t)
+
+
+def step_packed(solvers, nsteps):
+    """nsteps fused steps of each solver as packed launches (nat_step_packed): one launch per
+    step advances all of them; each result is bit-identical to its own step(nsteps)."""
+    solvers = list(solvers)
+    hs = (ctypes.c_void_p * len(solvers))(*[s._h for s in solvers])
+    rc = lib().nat_step_packed(hs, len(solvers), int(nsteps))
+    if rc:
+        msg = lib().nat_last_error(solvers[0]._h).decode() if solvers else ""
+        if rc == 4:
+            raise DensityZeroError(rc, msg)
+        raise NatError(rc, msg)
+
 
 # ----------------------------------------------------------------------------- request server
 PROBLEMS = {"TGV2D": 0, "TGV3D": 1, "SHOCK_TUBE": 2}
@@ -387,18 +419,19 @@ class _Result(ctypes.Structure):
     _fields_ = [
         ("status", ctypes.c_int), ("n_dofs", ctypes.c_int64), ("queue_ms", ctypes.c_double),
         ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double), ("mass", ctypes.c_double),
-        ("ekin", ctypes.c_double), ("footprint_bytes", ctypes.c_int64),
+        ("ekin", ctypes.c_double), ("footprint_bytes", ctypes.c_int64), ("error", ctypes.c_char * 256),
     ]
 
 
 class _ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_active", ctypes.c_int), ("chunk_steps", ctypes.c_int),
-                ("use_graphs", ctypes.c_int)]
+                ("use_graphs", ctypes.c_int), ("pack", ctypes.c_int), ("autotune", ctypes.c_int),
+                ("setup_threads", ctypes.c_int)]
 
 
 class _ServerStats(ctypes.Structure):
     _fields_ = [("queued", ctypes.c_int64), ("active", ctypes.c_int64), ("used_bytes", ctypes.c_int64),
-                ("node_updates", ctypes.c_int64), ("busy_ms", ctypes.c_double)]
+                ("node_updates", ctypes.c_int64), ("busy_ms", ctypes.c_double), ("packed_chunks", ctypes.c_int64)]
 
 
 SERVER_EXPORTS = ["nat_server_create", "nat_server_submit", "nat_server_wait", "nat_server_stats",
@@ -430,9 +463,11 @@ class Server:
     returns its timings, conservation sums and (optionally) its velocity field.
     """
 
-    def __init__(self, hbm_budget_bytes=0, max_active=8, chunk_steps=50, use_graphs=True):
+    def __init__(self, hbm_budget_bytes=0, max_active=8, chunk_steps=50, use_graphs=True, pack=True, autotune=True,
+                 setup_threads=4):
         L = _server_lib()
-        o = _ServerOpts(int(hbm_budget_bytes), int(max_active), int(chunk_steps), int(bool(use_graphs)))
+        o = _ServerOpts(int(hbm_budget_bytes), int(max_active), int(chunk_steps), int(bool(use_graphs)),
+                        int(bool(pack)), int(bool(autotune)), int(setup_threads))
         h = ctypes.c_void_p()
         rc = L.nat_server_create(ctypes.byref(o), ctypes.byref(h))
         if rc != 0:
@@ -456,6 +491,7 @@ class Server:
         rc = _server_lib().nat_server_wait(self._h, int(rid), ctypes.byref(res), _ptr(u) if u is not None else None,
                                            0 if u is None else u.size)
         out = {k: getattr(res, k) for k, _ in _Result._fields_}
+        out["error"] = res.error.decode(errors="replace")
         out["status"] = rc
         if u is not None:
             out["u"] = u.reshape(int(dim), int(n_dofs))

@@ -33,6 +33,9 @@
 //
 // Storage: unique periodic nodes (N_a p per axis, x fastest); the FE_Q DoFs on the
 // upper periodic faces are copies of the lower ones and are produced at download.
+#include <map>
+#include <mutex>
+
 #include "nat_collide.hpp"
 
 // directions per group (FP32 / FP64 storage) and the D3Q27 p=4 tile
@@ -50,6 +53,9 @@
 #endif
 #ifndef NAT_TILE_27_4
 #define NAT_TILE_27_4 16, 8, 8
+#endif
+#ifndef NAT_PIPE_PASSES
+#define NAT_PIPE_PASSES 0  // later passes of a group: all LDS reads first, then the sums
 #endif
 
 namespace nat {
@@ -238,6 +244,17 @@ struct StructCfg {
     return m;
   }
   static constexpr int KI = (first_items() + NT - 1) / NT;
+  // register slots for a later pass
+  static constexpr int later_items() {
+    int m = 1;
+    for (int al = 1; al < Q; al++)
+      for (int k = 1; k < 3; k++) {
+        const PG g(D, P, TX, TY, TZ, Lattice<Q>::e[al][0], Lattice<Q>::e[al][1], (D == 3) ? Lattice<Q>::e[al][2] : 0, k);
+        if (k < g.np && g.items > m) m = g.items;
+      }
+    return m;
+  }
+  static constexpr int KL = (later_items() + NT - 1) / NT;
 };
 
 // SYM: the host found the 1-D tables identical on all axes and mirror symmetric,
@@ -246,250 +263,82 @@ struct StructCfg {
 template <int D, int Q, int P, typename T, int TX, int TY, int TZ, int B, int WPE, bool SYM>
 __global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_sl_struct(
     StructArgs<T> a, const double* __restrict__ wtab, CollideArgs<T> ca, int write_macro) {
-  using Cfg = StructCfg<D, Q, P, TX, TY, TZ, B>;
-  static_assert(TX % P == 0 && TY % P == 0 && (D == 2 || TZ % P == 0), "tiles must be cell aligned");
-  constexpr int NT = Cfg::NT, KI = Cfg::KI;
-  constexpr int TW = (TX > TY ? (TX > TZ ? TX : TZ) : (TY > TZ ? TY : TZ)) + P + 1;
-  __shared__ double b1[B][Cfg::B1];
-  __shared__ double b2[B][Cfg::B2];
-  __shared__ T b3[2][B][Cfg::B3];
-  __shared__ int gt[3][TW];  // byte offsets of the wrapped window nodes [X0-P, X0+T] per axis
-
-  const unsigned tid = threadIdx.x;
-  // tile index, optionally regrouped so that consecutive tiles (which share halos)
-  // run on the same XCD: hardware dispatches workgroup i to XCD i % 8
   int bid = blockIdx.x;
-  const int nb = a.tiles[0] * a.tiles[1] * a.tiles[2];
-  if (a.xcd_remap) bid = (bid % 8) * (nb / 8) + bid / 8;
-  const int org[3] = {(bid % a.tiles[0]) * TX, ((bid / a.tiles[0]) % a.tiles[1]) * TY,
-                      (bid / (a.tiles[0] * a.tiles[1])) * TZ};
-  const int nx = a.n[0], ny = a.n[1], nz = a.n[2];
-  const unsigned ox0 = tid % TX, oy0 = (tid / TX) % TY, oz0 = tid / (TX * TY);
-  const bool active = (org[0] + (int)ox0 < nx) && (org[1] + (int)oy0 < ny) && (org[2] + (int)oz0 < nz);
-  const int node = (org[0] + ox0) + nx * ((org[1] + oy0) + ny * (org[2] + oz0));
-  T fs[Q];
-  fs[0] = active ? a.fin[node] : T(0);
-  const int astr[3] = {1, nx, nx * ny};
-  for (int i = tid; i < 3 * TW; i += NT) {
-    const int ax = i / TW, k = i % TW;
-    const int n = a.n[ax];
-    gt[ax][k] = (ax < D) ? (((org[ax] - P + k) % n + n) % n) * astr[ax] * (int)sizeof(T) : 0;
-  }
-
-  double W0[SYM ? P : 1][P + 1];
-  if constexpr (SYM) {
-#pragma unroll
-    for (int j = 0; j < P; j++)
-#pragma unroll
-      for (int m = 0; m <= P; m++) W0[j][m] = wtab[j * (MAXP + 1) + m];
-  }
-
-  // first-pass inputs of the next PF groups (register slot sets, rotating by group)
-  constexpr int PF = (sizeof(T) == 4) ? NAT_PREFETCH32 : NAT_PREFETCH64;
-  T xin[PF][B][KI][P + 1];
-  int xo0[PF][B][KI];         // output base of the first-pass item, computed with its loads
-  unsigned xsc[PF][B][KI];    // its source cell along the pass axis
-  // the offset tables are LDS: wait for LDS only, so the rest population's global load
-  // stays in flight (a full __syncthreads would wait for it here)
-  lds_barrier();
-
-  // raw buffer loads: 32-bit byte offsets against a per-direction resource (range checked)
-  auto load_t = [](__amdgpu_buffer_rsrc_t r, int off) -> T {
-    if constexpr (sizeof(T) == 4)
-      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-    else
-      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-  };
-
-  // The directions of a group start their items at different waves, so that a
-  // pass with fewer items than threads does not leave the same waves idle for all
-  // directions of the group.
-  auto rot = [](unsigned t, int b) -> unsigned {
-    const unsigned off = ((unsigned)(b * NT / B) / 64u) * 64u;
-    const unsigned r = t + off;
-    return r >= (unsigned)NT ? r - NT : r;
-  };
-
-  // ---- global loads of a direction's first pass into register slot b
-  auto load_first = [&](auto ALc, auto Bc, auto SETc) {
-    constexpr int al = decltype(ALc)::value, b = decltype(Bc)::value, ps = decltype(SETc)::value;
-    if constexpr (al > 0) {
-      constexpr PG G = PassGeo<D, Q, P, TX, TY, TZ, al, 0>::g;
-      constexpr int KK = (G.items + NT - 1) / NT;
-      static_assert(KK <= KI, "first-pass register slots");
-      // the base is uniform; readfirstlane lets the compiler keep the resource in SGPRs
-      // (readfirstlane returns int: both halves go through uint32_t, no sign extension)
-      const uint64_t pb = (uint64_t)(a.fin + (int64_t)al * a.stride);
-      const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pb >> 32));
-      const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pb);
-      const uint64_t pu = ((uint64_t)phi << 32) | (uint64_t)plo;
-      const __amdgpu_buffer_rsrc_t src = __builtin_amdgcn_make_buffer_rsrc(
-          (void*)pu, (short)0, __builtin_amdgcn_readfirstlane(a.dir_bytes), 0x00020000);
-      const unsigned t0 = rot(tid, b);
-#pragma unroll
-      for (int k = 0; k < KK; k++) {
-        const unsigned it = t0 + k * NT;
-        if (it < (unsigned)G.items) {
-          const unsigned r = FastDiv<G.E1, KK * NT>::div(it);
-          const unsigned i1 = it - r * G.E1;
-          const unsigned sc = FastDiv<G.E2, (KK * NT) / G.E1 + 1>::div(r);
-          const unsigned i2 = r - sc * G.E2;
-          const int base = gt[G.O1][i1 + win_lo(G.c[G.O1], P) + P] + gt[G.O2][i2 + win_lo(G.c[G.O2], P) + P];
-          const int* ga = &gt[G.A][sc * P + win_lo(G.C, P) + P];
-#pragma unroll
-          for (int m = 0; m <= P; m++) xin[ps][b][k][m] = load_t(src, base + ga[m]);
-          constexpr int sh = (G.C > 0) ? 1 - P : 0;
-          xo0[ps][b][k] = (int)(i1 * G.sout[G.O1] + i2 * G.sout[G.O2]) + ((int)sc * P + sh) * G.sout[G.A];
-          xsc[ps][b][k] = sc;
-        }
-      }
-    }
-  };
-
-  // ---- one 1-D pass of direction al (slot b, output parity par)
-  auto run_pass = [&](auto ALc, auto Kc, auto Bc, auto PARc) {
-#pragma clang fp contract(off)  // the one fma below is explicit: all instantiations round alike
-    constexpr int al = decltype(ALc)::value, kp = decltype(Kc)::value;
-    constexpr int b = decltype(Bc)::value, par = decltype(PARc)::value;
-    constexpr int ps = decltype(PARc)::grp % PF;  // register slot set of this group
-    constexpr PG G = PassGeo<D, Q, P, TX, TY, TZ, al, kp>::g;
-    constexpr int KK = (G.items + NT - 1) / NT;
-    // this pass's weight rows [class][m], uniform addresses: scalar loads
-    const double* __restrict__ wt = wtab + (G.A * MAXSHIFT + (G.C > 0 ? 0 : 1)) * MAXP * (MAXP + 1);
-    const unsigned t0 = rot(tid, b);
-#pragma unroll
-    for (int k = 0; k < KK; k++) {
-      const unsigned it = t0 + k * NT;
-      if (it < (unsigned)G.items) {
-        double v[P + 1];
-        int o0;
-        unsigned sc;
-        constexpr int sh = (G.C > 0) ? 1 - P : 0;
-        if constexpr (G.first) {
-#pragma unroll
-          for (int m = 0; m <= P; m++) v[m] = (double)xin[ps][b][k][m];
-          o0 = xo0[ps][b][k];
-          sc = xsc[ps][b][k];
-        } else {
-          const unsigned r = FastDiv<G.E1, KK * NT>::div(it);
-          const unsigned i1 = it - r * G.E1;
-          sc = FastDiv<G.E2, (KK * NT) / G.E1 + 1>::div(r);
-          const unsigned i2 = r - sc * G.E2;
-          const double* in = (kp == 1) ? b1[b] : b2[b];
-          const unsigned i0 = (i1 + G.inoff[G.O1]) * G.sin[G.O1] + (i2 + G.inoff[G.O2]) * G.sin[G.O2] +
-                              sc * (P * G.sin[G.A]);
-#pragma unroll
-          for (int m = 0; m <= P; m++) v[m] = in[i0 + m * G.sin[G.A]];
-          o0 = (int)(i1 * G.sout[G.O1] + i2 * G.sout[G.O2]) + ((int)sc * P + sh) * G.sout[G.A];
-        }
-        // the P accumulations are independent chains: interleave them (m outer)
-        double acc[P];
-#pragma unroll
-        for (int m = 0; m <= P; m++) {
-#pragma unroll
-          for (int kk = 0; kk < P; kk++) {
-            double w;
-            if constexpr (SYM)
-              w = (G.C > 0) ? W0[(kk + 1) % P][m] : W0[(P - kk) % P][P - m];
-            else
-              w = wt[((kk + (G.C > 0 ? 1 : 0)) % P) * (MAXP + 1) + m];
-            acc[kk] = (m == 0) ? w * v[0] : fma(w, v[m], acc[kk]);
-          }
-        }
-        // output kk is the node sc*P + kk (+1-P for a positive component) of the tile
-        const bool lo_ok = G.C < 0 || sc != 0;                      // outputs kk < P-1
-        const bool hi_ok = G.C < 0 || sc != (unsigned)(G.NC - 1);   // output kk = P-1
-#pragma unroll
-        for (int kk = 0; kk < P; kk++) {
-          if (kk < P - 1 ? lo_ok : hi_ok) {
-            if constexpr (G.last)
-              b3[par][b][o0 + kk * G.sout[G.A]] = (T)acc[kk];
-            else
-              ((kp == 0) ? b1[b] : b2[b])[o0 + kk * G.sout[G.A]] = acc[kk];
-          }
-        }
-      }
-    }
-  };
-
-  constexpr int NG = kSchedule<D, Q, B>.ngroups;
-  static_for<0, (PF < NG ? PF : NG)>([&](auto Gc) {
-    constexpr int g = decltype(Gc)::value;
-    static_for<0, B>([&](auto Bc) {
-      constexpr int b = decltype(Bc)::value;
-      load_first(std::integral_constant<int, kSchedule<D, Q, B>.dir[g][b]>{}, Bc, std::integral_constant<int, g % PF>{});
-    });
-  });
-
-  static_for<0, NG>([&](auto Gc) {
-    constexpr int g = decltype(Gc)::value;
-    constexpr int np = kSchedule<D, Q, B>.npass[g];
-    using PAR = GroupTag<g>;
-    // first passes (register inputs)
-    static_for<0, B>([&](auto Bc) {
-      constexpr int al = kSchedule<D, Q, B>.dir[g][decltype(Bc)::value];
-      if constexpr (al > 0) run_pass(std::integral_constant<int, al>{}, std::integral_constant<int, 0>{}, Bc, PAR{});
-    });
-    // the inputs of group g + PF go into the slot set just consumed; they are in flight
-    // during the remaining passes of this group and all of the next PF - 1 groups
-    if constexpr (g + PF < NG) {
-      static_for<0, B>([&](auto Bc) {
-        constexpr int b = decltype(Bc)::value;
-        load_first(std::integral_constant<int, kSchedule<D, Q, B>.dir[g + PF][b]>{}, Bc,
-                   std::integral_constant<int, g % PF>{});
-      });
-    }
-    lds_barrier();
-    static_for<1, np>([&](auto Kc) {
-      static_for<0, B>([&](auto Bc) {
-        constexpr int al = kSchedule<D, Q, B>.dir[g][decltype(Bc)::value];
-        if constexpr (al > 0) run_pass(std::integral_constant<int, al>{}, Kc, Bc, PAR{});
-      });
-      lds_barrier();
-    });
-    // pick up this group's values for the own node
-    static_for<0, B>([&](auto Bc) {
-      constexpr int b = decltype(Bc)::value;
-      constexpr int al = kSchedule<D, Q, B>.dir[g][b];
-      if constexpr (al > 0) {
-        constexpr PG G = PassGeo<D, Q, P, TX, TY, TZ, al, kSchedule<D, Q, B>.npass[g] - 1>::g;
-        constexpr unsigned off = (G.A == 0 ? G.outoff : 0) + G.sout[1] * (G.A == 1 ? G.outoff : 0) +
-                                 G.sout[2] * (G.A == 2 ? G.outoff : 0);
-        T val = b3[g & 1][b][off + ox0 + G.sout[1] * oy0 + G.sout[2] * oz0];
-        asm volatile("" : "+v"(val));
-        fs[al] = val;
-      }
-    });
-  });
-
-  if (!active) return;
-  if (write_macro & STEP_STREAM_ONLY) {  // the SL stream alone (CFDSolver::stream)
-#pragma unroll
-    for (int q = 0; q < Q; q++) a.fout[(int64_t)q * a.stride + node] = fs[q];
-    return;
-  }
-  bgk_collide_store_pairs<Q, T>(fs, ca, node, (write_macro & STEP_MACRO) != 0, a.fout, a.stride);
+#include "k_sl_struct_body.inc"
 }
 
-// query != null: report the tile (TX, TY, TZ) instead of launching
+
+// Packed steps: one launch advances several independent solves of the same instantiation
+// (stencil, order, storage, tile) by one step each. Solve r owns the workgroups
+// [block0[r], block0[r + 1]); each of its tiles runs exactly as in its own launch, so its
+// populations do not depend on the other members of the pack.
+template <typename T>
+struct PackArgs {  // kernel argument block: sizeof <= 4 KiB (static_assert below)
+  int n;
+  int block0[NAT_PACK_MAX + 1];
+  int write_macro[NAT_PACK_MAX];
+  StructArgs<T> a[NAT_PACK_MAX];
+  CollideArgs<T> ca[NAT_PACK_MAX];
+};
+
+static_assert(sizeof(PackArgs<double>) <= 4096, "packed step arguments exceed the kernel argument block");
+
+template <int D, int Q, int P, typename T, int TX, int TY, int TZ, int B, int WPE, bool SYM>
+__global__ __launch_bounds__(TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_sl_struct_packed(
+    PackArgs<T> pk) {
+  int r = 0;
+  while (r + 1 < pk.n && pk.block0[r + 1] <= (int)blockIdx.x) r++;  // uniform: scalar loads and branches
+  const StructArgs<T>& a = pk.a[r];
+  const double* __restrict__ wtab = a.w;
+  const CollideArgs<T>& ca = pk.ca[r];
+  const int write_macro = pk.write_macro[r];
+  int bid = (int)blockIdx.x - pk.block0[r];
+#include "k_sl_struct_body.inc"
+}
+
+// How a dispatch ends: report the tile, launch one solve, or launch a packed step.
+struct StructMode {
+  int* query = nullptr;        // != null: report the tile (TX, TY, TZ) and whether it is the small one
+  const void* pack = nullptr;  // != null: host PackArgs<T> of a packed step (passed by value)
+  int pack_blocks = 0;
+  hipStream_t pack_stream = nullptr;
+};
+
 template <int D, int Q, int P, typename T, int TX, int TY, int TZ, bool SYM_ONLY = false>
 static int launch_struct(nat_ctx* ctx, StructArgs<T> a, const CollideArgs<T>& ca, int write_macro,
-                         int* query) {
-  if (query) {
-    query[0] = TX;
-    query[1] = TY;
-    query[2] = TZ;
+                         const StructMode& md, int small = 0) {
+  // FP64 at p = 6: groups of 2 directions (3 hold 3 x 7 FP64 first-pass values per item
+  // slot and spill at the 128-register budget)
+  constexpr int B = sizeof(T) == 4 ? NAT_STRUCT_B32 : (P >= 6 ? 2 : NAT_STRUCT_B64);
+  constexpr int WPE = 4;  // 16 waves per CU: one 1024-thread tile, up to 128 VGPRs
+  if (md.query) {
+    md.query[0] = TX;
+    md.query[1] = TY;
+    md.query[2] = TZ;
+    md.query[3] = small;
+    md.query[4] = (SYM_ONLY || ctx->ax_sym) ? 1 : 0;
     return 0;
+  }
+  if (md.pack) {
+    // instantiated where the packed form keeps the one-solve kernel's registers (no SGPR or
+    // VGPR spills): symmetric tables, p = 2-4; pack_key admits only these
+    if constexpr (P >= 2 && P <= 4) {
+      if (SYM_ONLY || ctx->ax_sym) {
+        const PackArgs<T>& pk = *(const PackArgs<T>*)md.pack;
+        hipLaunchKernelGGL((k_sl_struct_packed<D, Q, P, T, TX, TY, TZ, B, WPE, true>), dim3(md.pack_blocks),
+                           dim3(TX * TY * TZ), 0, md.pack_stream, pk);
+        NAT_HIP(hipGetLastError());
+        return 0;
+      }
+    }
+    ctx->err = "packed structured step: not built for this (order, tables)";
+    return NAT_ERR_UNSUPPORTED;
   }
   a.tiles[0] = (a.n[0] + TX - 1) / TX;
   a.tiles[1] = (a.n[1] + TY - 1) / TY;
   a.tiles[2] = D == 3 ? (a.n[2] + TZ - 1) / TZ : 1;
   const int nb = a.tiles[0] * a.tiles[1] * a.tiles[2];
   a.xcd_remap = (nb % 8 == 0) ? 1 : 0;
-  // FP64 at p = 6: groups of 2 directions (3 hold 3 x 7 FP64 first-pass values per item
-  // slot and spill at the 128-register budget)
-  constexpr int B = sizeof(T) == 4 ? NAT_STRUCT_B32 : (P >= 6 ? 2 : NAT_STRUCT_B64);
-  constexpr int WPE = 4;  // 16 waves per CU: one 1024-thread tile, up to 128 VGPRs
   if (SYM_ONLY || ctx->ax_sym)
     hipLaunchKernelGGL((k_sl_struct<D, Q, P, T, TX, TY, TZ, B, WPE, true>), dim3(nb), dim3(TX * TY * TZ), 0,
                        ctx->stream, a, a.w, ca, write_macro);
@@ -504,26 +353,29 @@ static int launch_struct(nat_ctx* ctx, StructArgs<T> a, const CollideArgs<T>& ca
 // tile of ~256 nodes spreads the step over more CUs (a 16^3-cell p = 2 box has 32 tiles of
 // 16x8x8 but 128 of 8x8x4): 8^3-cell boxes at p = 2-4 step 20-40% faster. Instantiated for
 // the periodic boxes whose 1-D tables are symmetric (all uniform boxes) at p = 2, 3, 4; at
-// p = 6 a small tile is one cell plus its halo and measured slower.
+// p = 6 a small tile is one cell plus its halo and measured slower. The autotuner
+// (nat_autotune) may fix the choice per mesh instead (ctx->tile_pref).
 template <int D, int Q, int P, typename T, int TX, int TY, int TZ, int SX, int SY, int SZ>
-static int launch_struct_sized(nat_ctx* ctx, const StructArgs<T>& a, const CollideArgs<T>& ca, int wm, int* query) {
+static int launch_struct_sized(nat_ctx* ctx, const StructArgs<T>& a, const CollideArgs<T>& ca, int wm,
+                               const StructMode& md) {
   const int64_t nb = (int64_t)((a.n[0] + TX - 1) / TX) * ((a.n[1] + TY - 1) / TY) * (D == 3 ? (a.n[2] + TZ - 1) / TZ : 1);
-  if (ctx->ax_sym && nb < 256) return launch_struct<D, Q, P, T, SX, SY, SZ, true>(ctx, a, ca, wm, query);
-  return launch_struct<D, Q, P, T, TX, TY, TZ>(ctx, a, ca, wm, query);
+  const bool small = ctx->ax_sym && (ctx->tile_pref >= 0 ? ctx->tile_pref == 1 : nb < 256);
+  if (small) return launch_struct<D, Q, P, T, SX, SY, SZ, true>(ctx, a, ca, wm, md, 1);
+  return launch_struct<D, Q, P, T, TX, TY, TZ>(ctx, a, ca, wm, md, 0);
 }
 
 template <typename T>
-static int dispatch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideArgs<T>& ca, int wm, int* query) {
+static int dispatch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideArgs<T>& ca, int wm, const StructMode& md) {
   const int Q = ctx->Q, P = ctx->p;
 #define NAT_S3(QQ, PP, TX, TY, TZ) \
-  if (Q == QQ && P == PP) return launch_struct<3, QQ, PP, T, TX, TY, TZ>(ctx, a, ca, wm, query);
+  if (Q == QQ && P == PP) return launch_struct<3, QQ, PP, T, TX, TY, TZ>(ctx, a, ca, wm, md);
 #define NAT_S3S(QQ, PP, TX, TY, TZ, SX, SY, SZ) \
-  if (Q == QQ && P == PP) return launch_struct_sized<3, QQ, PP, T, TX, TY, TZ, SX, SY, SZ>(ctx, a, ca, wm, query);
+  if (Q == QQ && P == PP) return launch_struct_sized<3, QQ, PP, T, TX, TY, TZ, SX, SY, SZ>(ctx, a, ca, wm, md);
 #define NAT_S3X(...) NAT_S3S(__VA_ARGS__)
 #define NAT_S2(QQ, PP, TX, TY) \
-  if (Q == QQ && P == PP) return launch_struct<2, QQ, PP, T, TX, TY, 1>(ctx, a, ca, wm, query);
+  if (Q == QQ && P == PP) return launch_struct<2, QQ, PP, T, TX, TY, 1>(ctx, a, ca, wm, md);
 #define NAT_S2S(QQ, PP, TX, TY, SX, SY) \
-  if (Q == QQ && P == PP) return launch_struct_sized<2, QQ, PP, T, TX, TY, 1, SX, SY, 1>(ctx, a, ca, wm, query);
+  if (Q == QQ && P == PP) return launch_struct_sized<2, QQ, PP, T, TX, TY, 1, SX, SY, 1>(ctx, a, ca, wm, md);
   if (ctx->D == 3) {
     NAT_S3(27, 1, 16, 8, 8)
     NAT_S3S(27, 2, 16, 8, 8, 8, 8, 4)
@@ -551,16 +403,16 @@ static int dispatch_struct(nat_ctx* ctx, const StructArgs<T>& a, const CollideAr
   return NAT_ERR_UNSUPPORTED;
 }
 
+// kernel arguments of one step of ctx reading buffer src
 template <typename T>
-static int launch_typed(nat_ctx* ctx, int write_macro, int* query = nullptr) {
+static int make_args(nat_ctx* ctx, int src, StructArgs<T>& a, CollideArgs<T>& ca) {
   if (!ctx->ax_w || !ctx->ax_off) {
     ctx->err = "structured backend: no 1-D operator tables (the last assembly failed)";
     return NAT_ERR_STATE;
   }
-  const int src = ctx->cur, dst = 1 - ctx->cur;
-  StructArgs<T> a;
+  a = StructArgs<T>{};
   a.fin = (const T*)ctx->f[src];
-  a.fout = (T*)ctx->f[dst];
+  a.fout = (T*)ctx->f[1 - src];
   a.stride = ctx->stride;
   for (int i = 0; i < 3; i++) a.n[i] = ctx->nu[i];
   a.w = (const double*)ctx->ax_w;
@@ -569,7 +421,7 @@ static int launch_typed(nat_ctx* ctx, int write_macro, int* query = nullptr) {
     return NAT_ERR_UNSUPPORTED;
   }
   a.dir_bytes = (int)(ctx->stride * (int64_t)sizeof(T));
-  CollideArgs<T> ca;
+  ca = CollideArgs<T>{};
   ca.omega = (T)(1.0 / ctx->tau);
   ca.omega64 = 1.0 / ctx->tau;
   ca.rho0 = (T)(ctx->fp32 ? (ctx->desc.rho0 > 0 ? ctx->desc.rho0 : 1.0) : 0.0);
@@ -579,9 +431,88 @@ static int launch_typed(nat_ctx* ctx, int write_macro, int* query = nullptr) {
   ca.mstride = ctx->n_store;
   ca.status = ctx->d_status;
   ca.stab = ctx->stab_on();
-  int rc = dispatch_struct<T>(ctx, a, ca, write_macro, query);
-  if (rc == 0 && !query) ctx->cur = dst;
+  return 0;
+}
+
+template <typename T>
+static int launch_typed(nat_ctx* ctx, int write_macro, int* query = nullptr) {
+  StructArgs<T> a;
+  CollideArgs<T> ca;
+  if (int rc = make_args<T>(ctx, ctx->cur, a, ca)) return rc;
+  StructMode md;
+  md.query = query;
+  const int rc = dispatch_struct<T>(ctx, a, ca, write_macro, md);
+  if (rc == 0 && !query) ctx->cur = 1 - ctx->cur;
   return rc;
+}
+
+// ---- packed steps
+
+// Packing key of a structured context: members of one pack share the kernel
+// instantiation (dimension, stencil, order, storage, tile, weight symmetry).
+static int pack_key(nat_ctx* ctx, int64_t* key) {
+  if (ctx->desc.scheme != NAT_SL || ctx->desc.compressible || ctx->backend != NAT_BACKEND_STRUCTURED ||
+      !ctx->assembled || !ctx->ax_sym || ctx->p < 2 || ctx->p > 4)
+    return NAT_ERR_ARG;
+  int q[5] = {0, 0, 0, 0, 0};
+  const int rc = ctx->fp32 ? launch_typed<float>(ctx, 0, q) : launch_typed<double>(ctx, 0, q);
+  if (rc) return rc;
+  *key = ((((((int64_t)ctx->D * 64 + ctx->Q) * 16 + ctx->p) * 2 + (ctx->fp32 ? 1 : 0)) * 64 + q[0]) * 64 + q[1]) * 64 +
+         q[2];
+  *key = *key * 4 + q[4] * 2 + q[3];
+  return 0;
+}
+
+template <typename T>
+static int pack_step_typed(nat_ctx* const* ctxs, int n, int nsteps, hipStream_t stream) {
+  nat_ctx* ctx = ctxs[0];  // errors of the pack as a whole are reported on the first member
+  int q[5] = {0, 0, 0, 0, 0};
+  if (int rc = launch_typed<T>(ctx, 0, q)) return rc;
+  const int TX = q[0], TY = q[1], TZ = q[2];
+  PackArgs<T> tab[3];  // [0] even steps, [1] odd steps, [2] the last step (kernel arguments)
+  for (int t = 0; t < 3; t++) {
+    tab[t] = PackArgs<T>{};
+    tab[t].n = n;
+  }
+  int blocks = 0;
+  for (int i = 0; i < n; i++) {
+    nat_ctx* c = ctxs[i];
+    for (int t = 0; t < 3; t++) {
+      const int par = (t == 2) ? ((nsteps - 1) & 1) : t;
+      StructArgs<T> a;
+      CollideArgs<T> ca;
+      if (int rc = make_args<T>(c, c->cur ^ par, a, ca)) {
+        if (c != ctx) ctx->err = c->err;
+        return rc;
+      }
+      a.tiles[0] = (a.n[0] + TX - 1) / TX;
+      a.tiles[1] = (a.n[1] + TY - 1) / TY;
+      a.tiles[2] = c->D == 3 ? (a.n[2] + TZ - 1) / TZ : 1;
+      const int nb = a.tiles[0] * a.tiles[1] * a.tiles[2];
+      a.xcd_remap = (nb % 8 == 0) ? 1 : 0;
+      tab[t].a[i] = a;
+      tab[t].ca[i] = ca;
+      tab[t].write_macro[i] = (t == 2) ? STEP_MACRO : 0;
+      tab[t].block0[i] = blocks;
+      if (t == 2) blocks += nb;
+    }
+  }
+  for (int t = 0; t < 3; t++) tab[t].block0[n] = blocks;
+  StructMode md;
+  md.pack_blocks = blocks;
+  md.pack_stream = stream;
+  StructArgs<T> a0;
+  CollideArgs<T> c0;
+  if (int rc = make_args<T>(ctx, ctx->cur, a0, c0)) return rc;
+  for (int s = 0; s < nsteps; s++) {
+    md.pack = &tab[s == nsteps - 1 ? 2 : (s & 1)];
+    if (int rc = dispatch_struct<T>(ctx, a0, c0, 0, md)) return rc;
+  }
+  for (int i = 0; i < n; i++) {
+    ctxs[i]->cur ^= (nsteps & 1);
+    for (int st = 0; st < nsteps; st++) ctxs[i]->time += ctxs[i]->dt;  // as step by step
+  }
+  return 0;
 }
 
 }  // namespace nat
@@ -591,5 +522,127 @@ int nat_launch_sl_struct(nat_ctx* ctx, int write_macro) {
 }
 
 int nat_launch_sl_struct_tile(nat_ctx* ctx, int* tile) {
-  return ctx->fp32 ? nat::launch_typed<float>(ctx, 0, tile) : nat::launch_typed<double>(ctx, 0, tile);
+  int q[5] = {0, 0, 0, 0, 0};
+  const int rc = ctx->fp32 ? nat::launch_typed<float>(ctx, 0, q) : nat::launch_typed<double>(ctx, 0, q);
+  for (int i = 0; i < 3; i++) tile[i] = q[i];
+  return rc;
+}
+
+int nat_internal_pack_key(nat_ctx* ctx, int64_t* key) { return nat::pack_key(ctx, key); }
+
+// nsteps fused steps of every context in ctxs (1 <= n <= NAT_PACK_MAX, equal pack keys, each
+// context idle) as packed launches on stream; the per-solve arguments travel as kernel
+// arguments (at most NAT_PACK_MAX solves fit the 4 KiB argument block).
+int nat_internal_pack_step(nat_ctx* const* ctxs, int n, int nsteps, hipStream_t stream) {
+  if (n < 1 || n > NAT_PACK_MAX || nsteps < 1) return NAT_ERR_ARG;
+  int64_t k0 = 0;
+  if (int rc = nat::pack_key(ctxs[0], &k0)) return rc;
+  for (int i = 1; i < n; i++) {
+    int64_t k = 0;
+    if (nat::pack_key(ctxs[i], &k) || k != k0) {
+      ctxs[0]->err = "packed step: members of different kernel instantiations";
+      return NAT_ERR_ARG;
+    }
+  }
+  return ctxs[0]->fp32 ? nat::pack_step_typed<float>(ctxs, n, nsteps, stream)
+                       : nat::pack_step_typed<double>(ctxs, n, nsteps, stream);
+}
+
+// ---- autotuner
+
+namespace {
+
+struct TuneKey {
+  int D, Q, p, fp32, sym, n0, n1, n2;
+  bool operator<(const TuneKey& o) const {
+    const int a[8] = {D, Q, p, fp32, sym, n0, n1, n2}, b[8] = {o.D, o.Q, o.p, o.fp32, o.sym, o.n0, o.n1, o.n2};
+    for (int i = 0; i < 8; i++)
+      if (a[i] != b[i]) return a[i] < b[i];
+    return false;
+  }
+};
+std::mutex g_tune_mutex;
+std::map<TuneKey, int> g_tune_cache;  // winning tile preference per (instantiation, mesh)
+
+// device time of nsteps launches of the step kernel with the context's current tile choice;
+// each reads the current buffer and writes the other one without advancing the state
+template <typename T>
+int time_tile(nat_ctx* ctx, int nsteps, hipEvent_t e0, hipEvent_t e1, float* ms) {
+  nat::StructArgs<T> a;
+  nat::CollideArgs<T> ca;
+  if (int rc = nat::make_args<T>(ctx, ctx->cur, a, ca)) return rc;
+  ca.rho = nullptr;
+  ca.u = nullptr;
+  nat::StructMode md;
+  for (int s = 0; s < 2; s++)
+    if (int rc = nat::dispatch_struct<T>(ctx, a, ca, 0, md)) return rc;
+  NAT_HIP(hipEventRecord(e0, ctx->stream));
+  for (int s = 0; s < nsteps; s++)
+    if (int rc = nat::dispatch_struct<T>(ctx, a, ca, 0, md)) return rc;
+  NAT_HIP(hipEventRecord(e1, ctx->stream));
+  NAT_HIP(hipEventSynchronize(e1));
+  NAT_HIP(hipEventElapsedTime(ms, e0, e1));
+  return 0;
+}
+
+}  // namespace
+
+// Tile shape of the structured step kernel picked by measurement: every tile instantiated for
+// the context's (stencil, order) is timed for nsteps steps on the context's own mesh (the
+// launches write the idle ping-pong buffer; populations, fields and time stay as they were),
+// and the fastest is kept for this context and, process-wide, for every later context of the
+// same instantiation and mesh size, which then skips the timing. Results do not depend on the
+// tile: each node's interpolation and collision run the same arithmetic in any tile.
+int nat_internal_autotune(nat_ctx* ctx, int nsteps, int* tile, double* best_us) {
+  if (ctx->desc.scheme != NAT_SL || ctx->desc.compressible || ctx->backend != NAT_BACKEND_STRUCTURED) {
+    ctx->err = "nat_autotune: the tile autotuner serves the structured SL backend";
+    return NAT_ERR_ARG;
+  }
+  if (!ctx->assembled) return NAT_ERR_STATE;
+  const TuneKey key{ctx->D, ctx->Q, ctx->p, ctx->fp32 ? 1 : 0, ctx->ax_sym ? 1 : 0, ctx->nu[0], ctx->nu[1], ctx->nu[2]};
+  if (best_us) *best_us = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      ctx->tile_pref = it->second;
+      return nat_launch_sl_struct_tile(ctx, tile);
+    }
+  }
+  if (nsteps < 1) nsteps = 10;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  NAT_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    ctx->err = "hipEventCreate failed";
+    return NAT_ERR_HIP;
+  }
+  int best = -1, rc = 0;
+  float best_ms = 0;
+  for (int pref = 0; pref < 2 && !rc; pref++) {
+    ctx->tile_pref = pref;
+    int q[5] = {0, 0, 0, 0, 0};
+    rc = ctx->fp32 ? nat::launch_typed<float>(ctx, 0, q) : nat::launch_typed<double>(ctx, 0, q);
+    if (rc) break;
+    if (pref == 1 && q[3] == 0) break;  // no small tile for this instantiation
+    float ms = 0;
+    rc = ctx->fp32 ? time_tile<float>(ctx, nsteps, e0, e1, &ms) : time_tile<double>(ctx, nsteps, e0, e1, &ms);
+    if (!rc && (best < 0 || ms < best_ms)) {
+      best = pref;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    ctx->tile_pref = -1;
+    return rc;
+  }
+  ctx->tile_pref = best;
+  if (best_us) *best_us = 1e3 * best_ms / nsteps;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mutex);
+    g_tune_cache[key] = best;
+  }
+  return nat_launch_sl_struct_tile(ctx, tile);
 }

@@ -21,6 +21,9 @@ int nat_launch_assemble_axis(nat_ctx* ctx, const int* d_N, const double* d_vaxis
 int nat_launch_records_f32(nat_ctx* ctx, int64_t nrec);
 int nat_launch_sl_struct(nat_ctx* ctx, int write_macro);
 int nat_launch_sl_struct_tile(nat_ctx* ctx, int* tile);
+int nat_internal_pack_key(nat_ctx* ctx, int64_t* key);
+int nat_internal_pack_step(nat_ctx* const* ctxs, int n, int nsteps, hipStream_t stream);
+int nat_internal_autotune(nat_ctx* ctx, int nsteps, int* tile, double* best_us);
 int nat_launch_sl_general(nat_ctx* ctx, int write_macro);
 int nat_launch_collide(nat_ctx* ctx, int init);
 int nat_launch_stabilize(nat_ctx* ctx);
@@ -595,6 +598,7 @@ void nat_destroy(nat_ctx* ctx) {
   for (hipEvent_t e : ctx->ev)
     if (e) hipEventDestroy(e);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
+  if (ctx->cap_stream) hipStreamDestroy(ctx->cap_stream);
   delete ctx;
 }
 
@@ -660,6 +664,7 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
   ctx->tau = d.viscosity / (ctx->dt * ctx->st.cs2) + 0.5;
 
   NAT_HIP(hipStreamCreate(&ctx->stream));
+  NAT_HIP(hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking));
   TRY(alloc(ctx, &ctx->d_status, 1));
   TRY(alloc(ctx, &ctx->d_sums, 8));
   NAT_HIP(hipMemset(ctx->d_status, 0, sizeof(int)));
@@ -815,7 +820,7 @@ int nat_set_vertices(nat_ctx* ctx, const double* vertices) {
 }
 
 int nat_set_stabilizer(nat_ctx* ctx, int on) {
-  nat_drop_graphs(ctx);
+  if (ctx->stabilizer != (on != 0)) nat_drop_graphs(ctx);  // the flag is baked into captured steps
   ctx->stabilizer = on != 0;
   return 0;
 }
@@ -1069,29 +1074,48 @@ static int step_launch(nat_ctx* ctx, int nsteps, bool macro_last = true) {
 // stabilizer) drops the graphs when it changes.
 static constexpr int kGraphSteps = 32;
 
+// Captures nsteps fused steps (the last one writing the fields when macro_last) into a graph
+// without running them; time and the ping-pong state are left as they were. The launches are
+// recorded on the context's capture stream (see nat_ctx::cap_stream).
+extern "C++" int nat_internal_capture(nat_ctx* ctx, int nsteps, bool macro_last, hipGraph_t* out) {
+  *out = nullptr;
+  const double t = ctx->time;
+  const int par = ctx->cur;
+  hipStream_t run = ctx->stream;
+  ctx->stream = ctx->cap_stream;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  int rc = 0;
+  if (e == hipSuccess) {
+    rc = step_launch(ctx, nsteps, macro_last);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+    } else {
+      *out = g;
+    }
+  }
+  ctx->stream = run;
+  ctx->time = t;
+  ctx->cur = par;
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(ctx, NAT_ERR_HIP, std::string("step graph capture: ") + hipGetErrorString(e));
+  return 0;
+}
+
 static int step_enqueue(nat_ctx* ctx, int nsteps) {
   int left = nsteps;
   // the last step writes the macroscopic fields and always runs as a plain launch
   while (ctx->use_graphs && left - 1 >= kGraphSteps) {
     const int par = ctx->cur;
     if (!ctx->gexec[par]) {
-      const double t = ctx->time;
-      NAT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = step_launch(ctx, kGraphSteps, false);
       hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-      ctx->time = t;
-      ctx->cur = par;
-      if (rc) {
-        if (g) hipGraphDestroy(g);
-        return rc;
-      }
-      NAT_HIP(e);
+      TRY(nat_internal_capture(ctx, kGraphSteps, false, &g));
       ctx->graph[par] = g;
       NAT_HIP(hipGraphInstantiate(&ctx->gexec[par], g, nullptr, nullptr, 0));
     }
     NAT_HIP(hipGraphLaunch(ctx->gexec[par], ctx->stream));
-    ctx->time += kGraphSteps * ctx->dt;
+    for (int st = 0; st < kGraphSteps; st++) ctx->time += ctx->dt;  // as step by step
     left -= kGraphSteps;
   }
   return step_launch(ctx, left);
@@ -1215,6 +1239,64 @@ int nat_struct_tile(nat_ctx* ctx, int* tile) {
   if (ctx->desc.scheme != NAT_SL || ctx->backend != NAT_BACKEND_STRUCTURED || ctx->desc.compressible)
     return fail(ctx, NAT_ERR_ARG, "nat_struct_tile: not a structured SL context");
   return nat_launch_sl_struct_tile(ctx, tile);
+}
+
+int nat_pack_key(nat_ctx* ctx, int64_t* key) {
+  if (!key) return fail(ctx, NAT_ERR_ARG, "nat_pack_key: null key");
+  if (nat_internal_pack_key(ctx, key))
+    return fail(ctx, NAT_ERR_UNSUPPORTED,
+                "packed steps serve structured SL contexts (uniform periodic boxes) at p = 2-4");
+  return 0;
+}
+
+int nat_step_packed(nat_ctx* const* ctxs, int n, int nsteps) {
+  if (!ctxs || n < 1 || !ctxs[0]) return NAT_ERR_ARG;
+  nat_ctx* ctx = ctxs[0];
+  if (n > NAT_PACK_MAX) return fail(ctx, NAT_ERR_ARG, "nat_step_packed: more than 16 contexts");
+  if (nsteps < 0) return fail(ctx, NAT_ERR_ARG, "nat_step_packed: negative step count");
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i]) return fail(ctx, NAT_ERR_ARG, "nat_step_packed: null context");
+    if (ctxs[i]->device != ctx->device) return fail(ctx, NAT_ERR_ARG, "nat_step_packed: contexts on different GPUs");
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) return fail(ctx, NAT_ERR_ARG, "nat_step_packed: a context appears twice");
+  }
+  if (nsteps == 0) return 0;
+  // the launches go to the first context's stream, after the others' pending work; their
+  // later work waits for the launches
+  for (int i = 1; i < n; i++) {
+    NAT_HIP(hipEventRecord(ctxs[i]->ev[0], ctxs[i]->stream));
+    NAT_HIP(hipStreamWaitEvent(ctx->stream, ctxs[i]->ev[0], 0));
+  }
+  const int rc = nat_internal_pack_step(ctxs, n, nsteps, ctx->stream);
+  if (rc) {
+    if (ctx->err.empty()) ctx->err = "nat_step_packed: packing failed";
+    return rc;
+  }
+  NAT_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  for (int i = 1; i < n; i++) NAT_HIP(hipStreamWaitEvent(ctxs[i]->stream, ctx->ev[1], 0));
+  // each member's status word (density below the collision limit, ...)
+  for (int i = 0; i < n; i++) {
+    const int st = check_status(ctxs[i]);
+    if (st) {
+      if (i > 0) ctx->err = "context " + std::to_string(i) + ": " + ctxs[i]->err;
+      return st;
+    }
+  }
+  return 0;
+}
+
+int nat_autotune(nat_ctx* ctx, int nsteps, int* tile) {
+  nat_drop_graphs(ctx);  // captured steps hold the previous tile's launches
+  int t[3] = {0, 0, 0};
+  const int rc = nat_internal_autotune(ctx, nsteps, t, nullptr);
+  if (rc) return rc;
+  if (tile)
+    for (int i = 0; i < 3; i++) tile[i] = t[i];
+  // the timing launches leave the state untouched; a status bit they set (a context autotuned
+  // before its populations were initialised) is not an error of the solve
+  NAT_HIP(hipMemsetAsync(ctx->d_status, 0, sizeof(int), ctx->stream));
+  NAT_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
 }
 
 int nat_sync(nat_ctx* ctx) {

@@ -112,6 +112,9 @@ struct CollideArgs {
 // holds: 65536 x 5 doubles = 2.5 MiB per GPU, up to 16,777,216 pairs.
 #define NAT_SUMS_SCRATCH_BLOCKS 65536
 
+// Solves one packed structured launch can advance together.
+#define NAT_PACK_MAX 16
+
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
@@ -131,6 +134,10 @@ struct nat_ctx {
   bool collided = false;
   int backend = NAT_BACKEND_GENERAL;
   hipStream_t stream = nullptr;
+  // step graphs are captured here, not on `stream`: the context stream is a blocking stream,
+  // and a capture on it would be broken by any legacy-stream operation another thread makes
+  // meanwhile (a second context being set up)
+  hipStream_t cap_stream = nullptr;
   int device = -1;  // HIP device current at nat_create (the context's allocations live there)
 
   // device state
@@ -172,10 +179,11 @@ struct nat_ctx {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   bool use_graphs = true;
 
-  // residuals of the results table: fields of the previous 10th-iteration call
+  // residuals of the results table: fields of the previous 100th-iteration call
   std::vector<double> res_rho, res_u;
-  int64_t res_first = 0;
-  double res_rho_val = 0, res_u_val = 0;
+  int64_t res_first = 0, res_last = 0;
+  bool res_have = false;
+  double res_rho_val = 1.0, res_u_val = 1.0;  // the solver's start values (CFDSolver.cpp:398-399)
 
   // pseudo-entropic stabilizer as a data processor after each loop step's collision
   // (D2Q9 and D3Q19 only, as in the reference)
@@ -186,6 +194,7 @@ struct nat_ctx {
   void* ax_w = nullptr;   // [3][MAXSHIFT][MAXP][MAXP+1] weights (T)
   int* ax_off = nullptr;  // [3][MAXSHIFT][MAXP] source cell offsets
   bool ax_sym = false;    // tables equal on all axes and mirror symmetric in the shift sign
+  int tile_pref = -1;     // structured tile: -1 size heuristic, 0 large tile, 1 small tile (autotuner)
   int sid[nat::MAXQ][3] = {};
   int off_lo[3][nat::MAXSHIFT] = {}, off_hi[3][nat::MAXSHIFT] = {};
   int nshift[3] = {0, 0, 0};

@@ -242,9 +242,11 @@ int nat_stats(nat_ctx* ctx, int64_t iteration, nat_stats_t* out) {
   s.entropy = ent;
   s.mean_ux = ux_sum / area;
 
-  // residuals every 10th iteration against the fields of the previous such call, once 100
-  // iterations have passed since the first (SolverStats::calculateResiduals)
-  if (iteration % 10 == 0) {
+  // residuals at every 100th iteration against the fields of the previous such call, once
+  // 100 iterations have passed since the first (SolverStats::calculateResiduals, called by the
+  // stop check every 100 iterations, CFDSolver.cpp:927-931); a second call at the same
+  // iteration keeps them
+  if (iteration % 100 == 0 && !(ctx->res_have && iteration == ctx->res_last)) {
     if (!ctx->res_rho.empty() && iteration - ctx->res_first >= 100) {
       double rr = 0, ru = 0;
       for (int64_t i = 0; i < n; i++) {
@@ -262,6 +264,8 @@ int nat_stats(nat_ctx* ctx, int64_t iteration, nat_stats_t* out) {
     if (ctx->res_rho.empty()) ctx->res_first = iteration;
     ctx->res_rho = rho;
     ctx->res_u = u;
+    ctx->res_have = true;
+    ctx->res_last = iteration;
   }
   s.res_rho = ctx->res_rho_val;
   s.res_u = ctx->res_u_val;

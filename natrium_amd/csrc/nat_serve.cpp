@@ -4,24 +4,32 @@
 // (problem, mesh refinement, order, stencil, Reynolds and Mach number, step count,
 // precision) are queued by nat_server_submit from any thread. A scheduler thread
 //   - admits queued requests in arrival order while their device footprint fits
-//     the HBM budget and an active slot is free, building the context (mesh,
-//     operator, equilibrium initialisation, initial collision) on its own stream;
-//   - advances every active request by chunks of fused steps, each chunk a
-//     replay of a HIP graph captured once per request (or plain launches for the
-//     remainder), so the work of different requests overlaps on the device;
+//     the HBM budget and an active slot is free, and hands them to setup threads,
+//     which build the context (mesh, operator, equilibrium initialisation, initial
+//     collision, and for structured meshes the autotuned tile) on its own stream
+//     while the scheduler keeps the device busy with the others;
+//   - advances every active request by chunks of fused steps. Idle requests that
+//     share a packing key (structured SL, same kernel instantiation) advance
+//     together: one launch per step covers the tiles of all of them (up to 16), so
+//     small solves fill the GPU instead of leaving most CUs idle. Others run alone
+//     on their own stream, each chunk a replay of a HIP graph captured once per
+//     request (or plain launches for the remainder);
 //   - retires a request when its last chunk completes: status check, FP64
 //     conservation sums, velocity field copied to the host, context freed, and
 //     the freed memory immediately admits the next request.
-// Each request owns its context and stream, so its results do not depend on
-// which other requests share the device.
+// A packed launch runs each member's tiles exactly as its own launch would, and
+// each request owns its context, so its results do not depend on which other
+// requests share the device.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -30,8 +38,11 @@
 #include "nat_internal.hpp"
 
 int nat_internal_enqueue(nat_ctx* ctx, int nsteps);
+int nat_internal_capture(nat_ctx* ctx, int nsteps, bool macro_last, hipGraph_t* out);
 int nat_internal_status(nat_ctx* ctx);
 int64_t nat_internal_footprint(const nat_desc* d);
+int nat_internal_pack_key(nat_ctx* ctx, int64_t* key);
+int nat_internal_pack_step(nat_ctx* const* ctxs, int n, int nsteps, hipStream_t stream);
 
 namespace {
 
@@ -56,6 +67,8 @@ struct Req {
   hipEvent_t ev = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bool packable = false;
+  int64_t key = 0;  // packing key (nat_pack_key) when packable
   // results
   int64_t n = 0;
   double mass = 0, ekin = 0;
@@ -64,12 +77,29 @@ struct Req {
 
 }  // namespace
 
+// a stream for packed chunks; busy until its last chunk's event has completed
+struct PackSlot {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  bool busy = false;
+};
+
 struct nat_server {
   nat_server_opts o{};
   std::mutex m;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_setup;
   std::deque<Req*> queued;
+  std::deque<Req*> to_setup;  // admitted, waiting for a setup thread
+  std::vector<Req*> ready;    // built, waiting to join the active set
+  int n_setup = 0;            // admitted requests not yet active (to_setup, being built, ready)
   std::vector<Req*> active;
+  std::vector<std::thread> setup_th;
+  std::vector<PackSlot> slots;
+  // HIP invalidates a stream capture when another thread meanwhile makes a call that
+  // synchronises with the legacy default stream (set-up copies, launches on blocking streams),
+  // and fails that call too: set-ups hold this shared, a capture holds it exclusively and is
+  // skipped (plain launches for that chunk) while a set-up runs
+  std::shared_mutex cap_mu;
   std::unordered_map<int64_t, std::unique_ptr<Req>> all;
   int64_t next_id = 1;
   int64_t used = 0;
@@ -77,6 +107,7 @@ struct nat_server {
   std::thread th;
   // totals
   int64_t node_updates = 0;
+  int64_t packed_chunks = 0;
   double busy_t0 = -1, busy_t1 = 0;
 };
 
@@ -155,6 +186,8 @@ void initial_fields(const Req& q, const std::vector<double>& xyz, std::vector<do
   }
 }
 
+void finish(nat_server* s, Req& q);
+
 int setup(nat_server* s, Req& q) {
   int rc = nat_create(&q.desc, &q.ctx);
   if (rc) {
@@ -168,33 +201,69 @@ int setup(nat_server* s, Req& q) {
   initial_fields(q, xyz, rho, u);
   if ((rc = nat_init_equilibrium(c, rho.data(), u.data(), nullptr))) return rc;
   if ((rc = nat_collide(c))) return rc;
+  if (c->desc.scheme == NAT_SL && c->backend == NAT_BACKEND_STRUCTURED && !c->desc.compressible) {
+    if (s->o.autotune && (rc = nat_autotune(c, 10, nullptr))) return rc;
+    q.packable = s->o.pack && nat_internal_pack_key(c, &q.key) == 0;
+  }
   if (hipEventCreateWithFlags(&q.ev, hipEventDisableTiming) != hipSuccess) return NAT_ERR_HIP;
   q.remaining = q.r.nsteps;
-  (void)s;
   return 0;
 }
 
-// enqueue the next chunk of steps of an active request on its stream
-int issue(nat_server* s, Req& q) {
+// setup threads: build admitted requests while the scheduler steps the active ones
+void setup_worker(nat_server* s) {
+  for (;;) {
+    Req* q = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(s->m);
+      // once stopping, admission only drains the queue: with both empty nothing is left to build
+      s->cv_setup.wait(lk, [&] { return !s->to_setup.empty() || (s->stop && s->queued.empty()); });
+      if (s->to_setup.empty()) return;
+      q = s->to_setup.front();
+      s->to_setup.pop_front();
+    }
+    {
+      std::shared_lock<std::shared_mutex> cap(s->cap_mu);
+      q->rc = setup(s, *q);
+      if (q->rc) {
+        if (q->err.empty() && q->ctx) q->err = nat_last_error(q->ctx);
+        finish(s, *q);
+      }
+    }
+    if (q->rc) {
+      std::lock_guard<std::mutex> lk(s->m);
+      s->n_setup--;
+    } else {
+      std::lock_guard<std::mutex> lk(s->m);
+      s->ready.push_back(q);
+    }
+    s->cv_work.notify_one();
+  }
+}
+
+// enqueue the next chunk of steps of an active request (on the context's non-blocking
+// stream, see issue())
+int issue_chunk(nat_server* s, Req& q) {
   nat_ctx* c = q.ctx;
   const int chunk = s->o.chunk_steps > 0 ? s->o.chunk_steps : 50;
   int k = q.remaining < chunk ? q.remaining : chunk;
-  if (s->o.use_graphs && k == chunk && chunk % 2 == 0) {
-    if (!q.exec) {
-      // capture one chunk: an even number of ping-pong steps returns to the same buffers
-      const double t = c->time;
-      const int cur = c->cur;
-      if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return NAT_ERR_HIP;
-      int rc = nat_internal_enqueue(c, chunk);
+  bool graph = s->o.use_graphs && k == chunk && chunk % 2 == 0;
+  if (graph && !q.exec) {
+    // capture one chunk (an even number of ping-pong steps returns to the same buffers),
+    // unless a set-up is running (then this chunk goes as plain launches)
+    std::unique_lock<std::shared_mutex> cap(s->cap_mu, std::try_to_lock);
+    if (cap.owns_lock()) {
       hipGraph_t g = nullptr;
-      if (hipStreamEndCapture(c->stream, &g) != hipSuccess || rc) return rc ? rc : NAT_ERR_HIP;
+      if (int rc = nat_internal_capture(c, chunk, true, &g)) return rc;
       q.graph = g;
       if (hipGraphInstantiate(&q.exec, g, nullptr, nullptr, 0) != hipSuccess) return NAT_ERR_HIP;
-      c->time = t;
-      c->cur = cur;
+    } else {
+      graph = false;
     }
+  }
+  if (graph) {
     if (hipGraphLaunch(q.exec, c->stream) != hipSuccess) return NAT_ERR_HIP;
-    c->time += chunk * c->dt;
+    for (int st = 0; st < chunk; st++) c->time += c->dt;  // as step by step
   } else {
     int rc = nat_internal_enqueue(c, k);
     if (rc) return rc;
@@ -203,6 +272,50 @@ int issue(nat_server* s, Req& q) {
   if (hipEventRecord(q.ev, c->stream) != hipSuccess) return NAT_ERR_HIP;
   q.in_flight = true;
   return 0;
+}
+
+// Chunks run on the context's non-blocking stream, so the legacy-stream copies of other
+// requests' set-ups do not wait for them; everything else (set-up, retirement) uses the
+// context's own stream once the chunk's event has completed.
+int issue(nat_server* s, Req& q) {
+  nat_ctx* c = q.ctx;
+  hipStream_t own = c->stream;
+  c->stream = c->cap_stream;
+  const int rc = issue_chunk(s, q);
+  c->stream = own;
+  return rc;
+}
+
+// enqueue one packed chunk of the idle requests qs (equal keys, 2..NAT_PACK_MAX) on a free
+// pack slot; false when no slot is free or the pack cannot be launched (the caller then
+// issues them alone)
+bool issue_pack(nat_server* s, const std::vector<Req*>& qs) {
+  PackSlot* slot = nullptr;
+  for (PackSlot& sl : s->slots)
+    if (!sl.busy) {
+      slot = &sl;
+      break;
+    }
+  if (!slot) return false;
+  const int chunk = s->o.chunk_steps > 0 ? s->o.chunk_steps : 50;
+  int k = chunk;
+  std::vector<nat_ctx*> cs;
+  for (Req* q : qs) {
+    k = q->remaining < k ? q->remaining : k;
+    cs.push_back(q->ctx);
+  }
+  if (k < 1) return false;
+  if (nat_internal_pack_step(cs.data(), (int)cs.size(), k, slot->stream)) return false;
+  for (Req* q : qs) {
+    q->remaining -= k;
+    (void)hipEventRecord(q->ev, slot->stream);
+    q->in_flight = true;
+  }
+  (void)hipEventRecord(slot->ev, slot->stream);
+  slot->busy = true;
+  std::lock_guard<std::mutex> lk(s->m);
+  s->packed_chunks++;
+  return true;
 }
 
 void finish(nat_server* s, Req& q) {
@@ -237,41 +350,61 @@ void finish(nat_server* s, Req& q) {
 void run(nat_server* s) {
   const int max_active = s->o.max_active > 0 ? s->o.max_active : 8;
   for (;;) {
-    std::vector<Req*> admit;
     {
       std::unique_lock<std::mutex> lk(s->m);
       for (;;) {
-        while (!s->queued.empty() && (int)(s->active.size() + admit.size()) < max_active &&
-               (s->used + s->queued.front()->bytes <= s->o.hbm_budget_bytes || (s->active.empty() && admit.empty()))) {
+        bool admitted = false;
+        while (!s->queued.empty() && (int)s->active.size() + s->n_setup < max_active &&
+               (s->used + s->queued.front()->bytes <= s->o.hbm_budget_bytes ||
+                (s->active.empty() && s->n_setup == 0))) {
           Req* q = s->queued.front();
           s->queued.pop_front();
           s->used += q->bytes;
           q->state = 1;
           q->t_start = now_ms();
           if (s->busy_t0 < 0) s->busy_t0 = q->t_start;
-          admit.push_back(q);
+          s->to_setup.push_back(q);
+          s->n_setup++;
+          admitted = true;
         }
-        if (!admit.empty() || !s->active.empty()) break;
-        if (s->stop) return;
-        s->cv_work.wait(lk);
+        if (admitted) s->cv_setup.notify_all();
+        for (Req* q : s->ready) s->active.push_back(q);
+        s->n_setup -= (int)s->ready.size();
+        s->ready.clear();
+        if (!s->active.empty()) break;
+        if (s->stop && s->queued.empty() && s->n_setup == 0) return;
+        s->cv_work.wait_for(lk, std::chrono::milliseconds(s->n_setup ? 1 : 50));
       }
     }
-    for (Req* q : admit) {
-      q->rc = setup(s, *q);
-      if (q->rc) {
-        if (q->err.empty() && q->ctx) q->err = nat_last_error(q->ctx);
-        finish(s, *q);
-      } else {
-        std::lock_guard<std::mutex> lk(s->m);
-        s->active.push_back(q);
+    // free pack slots whose chunk has completed
+    for (PackSlot& sl : s->slots)
+      if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) sl.busy = false;
+    // keep every active request busy: packable idle requests by key, together
+    std::vector<std::pair<int64_t, std::vector<Req*>>> groups;
+    for (Req* q : s->active) {
+      if (q->in_flight || q->remaining <= 0 || q->rc) continue;
+      if (q->packable) {
+        auto g = groups.begin();
+        while (g != groups.end() && g->first != q->key) ++g;
+        if (g == groups.end()) {
+          groups.emplace_back(q->key, std::vector<Req*>());
+          g = groups.end() - 1;
+        }
+        if (g->second.size() < (size_t)NAT_PACK_MAX) {
+          g->second.push_back(q);
+          continue;
+        }
       }
+      q->rc = issue(s, *q);
+      if (q->rc) q->remaining = 0;
     }
-    // keep every active request busy
-    for (Req* q : s->active)
-      if (!q->in_flight && q->remaining > 0) {
+    for (auto& g : groups) {
+      if (g.second.size() >= 2 && issue_pack(s, g.second)) continue;
+      for (Req* q : g.second) {
         q->rc = issue(s, *q);
         if (q->rc) q->remaining = 0;
       }
+    }
     // retire completed requests
     bool progressed = false;
     for (size_t i = 0; i < s->active.size();) {
@@ -311,6 +444,22 @@ int nat_server_create(const nat_server_opts* opts, nat_server** out) {
     }
     s->o.hbm_budget_bytes = (int64_t)(0.8 * (double)fr);
   }
+  for (int i = 0; i < 4; i++) {  // pack streams (four hardware queues per process)
+    PackSlot sl;
+    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+      for (PackSlot& x : s->slots) {
+        (void)hipStreamDestroy(x.stream);
+        (void)hipEventDestroy(x.ev);
+      }
+      if (sl.stream) (void)hipStreamDestroy(sl.stream);
+      delete s;
+      return NAT_ERR_HIP;
+    }
+    s->slots.push_back(sl);
+  }
+  const int nthreads = s->o.setup_threads > 0 ? s->o.setup_threads : 4;
+  for (int i = 0; i < nthreads; i++) s->setup_th.emplace_back(setup_worker, s);
   s->th = std::thread(run, s);
   *out = s;
   return 0;
@@ -353,6 +502,7 @@ int nat_server_wait(nat_server* s, int64_t id, nat_result* res, double* u, int64
     res->mass = q->mass;
     res->ekin = q->ekin;
     res->footprint_bytes = q->bytes;
+    std::snprintf(res->error, sizeof(res->error), "%s", q->rc ? q->err.c_str() : "");
   }
   if (u && u_len > 0) {
     const int64_t m = (int64_t)q->u.size() < u_len ? (int64_t)q->u.size() : u_len;
@@ -370,6 +520,7 @@ int nat_server_stats(nat_server* s, nat_server_stats_t* st) {
   st->used_bytes = s->used;
   st->node_updates = s->node_updates;
   st->busy_ms = s->busy_t0 < 0 ? 0.0 : s->busy_t1 - s->busy_t0;
+  st->packed_chunks = s->packed_chunks;
   return 0;
 }
 
@@ -379,8 +530,21 @@ void nat_server_destroy(nat_server* s) {
     std::lock_guard<std::mutex> lk(s->m);
     s->stop = true;
     s->cv_work.notify_all();
+    s->cv_setup.notify_all();
   }
+  // the scheduler returns once every queued request has been built and retired
   if (s->th.joinable()) s->th.join();
+  {
+    std::lock_guard<std::mutex> lk(s->m);
+    s->cv_setup.notify_all();
+  }
+  for (std::thread& t : s->setup_th)
+    if (t.joinable()) t.join();
+  for (PackSlot& sl : s->slots) {
+    (void)hipStreamSynchronize(sl.stream);
+    (void)hipStreamDestroy(sl.stream);
+    (void)hipEventDestroy(sl.ev);
+  }
   delete s;
 }
 

@@ -3,6 +3,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import natrium_amd as nat
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -31,27 +33,29 @@ def test_binding_loads_without_gpu():
     assert L.nat_last_error(None) == b"null context"
 
 
-def test_descriptor_layout_matches_header(tmp_path):
-    """The ctypes mirror of nat_desc has the C struct's size and field offsets."""
+@pytest.mark.parametrize("cname,pyname", [("nat_desc", "_Desc"), ("nat_stats_t", "_Stats"),
+                                          ("nat_request", "_Request"), ("nat_result", "_Result"),
+                                          ("nat_server_opts", "_ServerOpts"), ("nat_server_stats_t", "_ServerStats")])
+def test_struct_layouts_match_header(tmp_path, cname, pyname):
+    """Each ctypes mirror of a C struct of the header has the C struct's size and field offsets."""
     import shutil
     import subprocess
-
-    import pytest
 
     cc = shutil.which("gcc")
     if cc is None:
         pytest.skip("no C compiler")
-    fields = [f[0] for f in nat._Desc._fields_]
-    prog = tmp_path / "desc.c"
-    body = "".join(f'  printf("%zu\\n", offsetof(nat_desc, {f}));\n' for f in fields)
+    cls = getattr(nat, pyname)
+    fields = [f[0] for f in cls._fields_]
+    prog = tmp_path / "layout.c"
+    body = "".join(f'  printf("%zu\\n", offsetof({cname}, {f}));\n' for f in fields)
     prog.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "natrium_amd.h"\nint main(void) {\n'
-                    '  printf("%zu\\n", sizeof(nat_desc));\n' + body + "  return 0;\n}\n")
-    exe = tmp_path / "desc"
+                    f'  printf("%zu\\n", sizeof({cname}));\n' + body + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
     subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
     out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
-    assert out[0] == ctypes.sizeof(nat._Desc)
+    assert out[0] == ctypes.sizeof(cls)
     for f, off in zip(fields, out[1:]):
-        assert getattr(nat._Desc, f).offset == off, f
+        assert getattr(cls, f).offset == off, f
 
 
 def test_walls_rejected_before_device_use():

@@ -2,9 +2,10 @@
 
 A request's result must not depend on the other requests in flight: the same
 request run alone and inside a mix of concurrently resident requests gives
-bit-identical velocity fields (every request owns its context and stream, and the
-step kernels are deterministic). Graph replays of captured step chunks must match
-plain launches bit for bit.
+bit-identical velocity fields (every request owns its context, packed launches run
+each member's tiles as its own launch would, and the step kernels are
+deterministic). Graph replays of captured step chunks must match plain launches bit
+for bit.
 """
 import numpy as np
 import pytest
@@ -53,7 +54,7 @@ def test_server_single_request_matches_direct_solver():
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_server_result_same_alone_and_in_mix(precision):
     alone = _run_alone(PROBE, precision=precision)
-    assert alone["status"] == 0
+    assert alone["status"] == 0, alone["error"]
     mix = [
         dict(problem="TGV3D", stencil="D3Q27", refinement=3, order=2, reynolds=400.0, mach=0.1, nsteps=60),
         dict(problem="SHOCK_TUBE", stencil="D2Q9", refinement=5, order=3, reynolds=100.0, mach=0.1, nsteps=90),
@@ -72,8 +73,8 @@ def test_server_result_same_alone_and_in_mix(precision):
         st = srv.stats()
     finally:
         srv.close()
-    assert inmix["status"] == 0
-    assert all(o["status"] == 0 for o in others), others
+    assert inmix["status"] == 0, inmix["error"]
+    assert all(o["status"] == 0 for o in others), [(o["status"], o["error"]) for o in others]
     np.testing.assert_array_equal(inmix["u"], alone["u"])
     assert inmix["mass"] == alone["mass"] and inmix["ekin"] == alone["ekin"]  # fixed-order reductions
     assert st["used_bytes"] == 0 and st["queued"] == 0 and st["active"] == 0
@@ -150,9 +151,11 @@ def test_server_c5_slice_same_alone_and_in_mix(precision):
         st = srv.stats()
     finally:
         srv.close()
-    bad = [(reqs[k], o["status"]) for k, o in enumerate(outs) if o["status"] != 0]
+    bad = [(reqs[k], o["status"], o["error"]) for k, o in enumerate(outs) if o["status"] != 0]
     assert not bad, bad
     assert st["used_bytes"] == 0 and st["queued"] == 0 and st["active"] == 0
+    print(f"C5 slice {precision}: {st['packed_chunks']} packed chunks", flush=True)
+    assert st["packed_chunks"] > 0  # structured requests of one instantiation advanced together
     for k in probes:
         alone = _run_alone(reqs[k], precision=precision)
         assert alone["status"] == 0
