@@ -17,6 +17,8 @@ import re
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -235,6 +237,44 @@ def extra_configs(nat, steps_sl=100, steps_err=500, cpu=False):
     return out
 
 
+def side_config(nat, name):
+    """(make(precision), init(solver), algorithmic FP32 bytes per step) of a side configuration,
+    for profiling runs: C3 on the general SL backend or with SEDG + RK5, or C4."""
+    from natrium_amd.problems import (box_vertices, divergence_free_field, lbm_parameters, perturbed_vertices,
+                                      riemann_quadrant_fields, riemann_states)
+
+    cs, s = lbm_parameters(1.0, 0.05)
+    L = 2 * math.pi
+    if name in ("c3_sl", "c3_sedg"):
+        v3 = perturbed_vertices(3, [24] * 3, [0] * 3, [L] * 3, amplitude=0.1, seed=1234)
+        scheme = "SL" if name == "c3_sl" else "SEDG"
+
+        def make(prec):
+            return nat.Solver("D3Q19", 3, [24] * 3, v3, cartesian=False, scaling=s, viscosity=0.01, cfl=0.4,
+                              precision=prec, scheme=scheme)
+
+        def init(sol):
+            u = divergence_free_field(sol.coords(), modes=8, amplitude=0.05 * cs, seed=1234)
+            sol.init_equilibrium(np.ones(sol.n), u)
+            sol.collide()
+
+        if name == "c3_sl":  # populations read and written once, plus one 16-byte record per (DoF, moving direction)
+            return make, init, lambda sol: sol.n * (2 * 19 * 4 + 18 * 16)
+        return make, init, lambda sol: sol.n * 5 * 4 * 18 * 4  # five stages: f, df read and written
+    v4 = box_vertices(2, [128, 128], [0, 0], [1, 1])
+
+    def make4(prec):
+        return nat.Solver("D2Q25H", 2, [128, 128], v4, scaling=2.0, viscosity=1e-3, cfl=0.4, compressible=True,
+                          gamma=1.4, precision=prec, periodic=[0, 0])
+
+    def init4(sol):
+        rho, u, T = riemann_quadrant_fields(sol.coords(), riemann_states(7))
+        sol.init_equilibrium(rho, u, T)
+        sol.collide()
+
+    return make4, init4, lambda sol: sol.n * (2 * 2 * 25 * 4 + 8)
+
+
 def serving_requests(n, seed, full):
     """The seeded serving mix (natrium_amd.problems.serving_mix)."""
     from natrium_amd.problems import serving_mix
@@ -311,7 +351,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the C3/C4 side measurements")
     ap.add_argument("--profile-steps", type=int, default=0,
-                    help="profiling run: only the FP32-shifted C2 fused kernel for this many steps")
+                    help="profiling run: only the FP32-shifted fused kernel of --profile-config for this many steps")
+    ap.add_argument("--profile-config", default="c2", choices=["c2", "c3_sl", "c3_sedg", "c4"],
+                    help="workload of --profile-steps: C2 (headline), C3 general SL or SEDG, C4 compressible")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="profiling run: a plain device copy of 1 GiB, ten times (counter calibration)")
     args = ap.parse_args()
@@ -337,12 +379,21 @@ def main():
 
     prob = tgv3d_problem()
     if args.profile_steps:
-        s32 = make_solver(nat, prob, "fp32")
-        init_tgv(s32, prob)
-        s32.step(args.profile_steps)
-        s32.sync()
-        print(json.dumps({"profile_steps": args.profile_steps, "nodes": s32.n_stored,
-                          "algorithmic_bytes_per_launch": s32.n_stored * 2 * 27 * 4}))
+        if args.profile_config == "c2":
+            s32 = make_solver(nat, prob, "fp32")
+            init_tgv(s32, prob)
+            s32.step(args.profile_steps)
+            s32.sync()
+            print(json.dumps({"profile_steps": args.profile_steps, "nodes": s32.n_stored,
+                              "algorithmic_bytes_per_launch": s32.n_stored * 2 * 27 * 4}))
+        else:
+            make, init, nbytes = side_config(nat, args.profile_config)
+            sol = make("fp32")
+            init(sol)
+            sol.step(args.profile_steps)
+            sol.sync()
+            print(json.dumps({"profile_config": args.profile_config, "profile_steps": args.profile_steps,
+                              "dofs": sol.n, "algorithmic_bytes_per_step": nbytes(sol)}))
         return
     Q = 27
     workload = "tgv3d_16c_p4_d3q27"

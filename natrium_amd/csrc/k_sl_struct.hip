@@ -54,9 +54,7 @@
 #ifndef NAT_TILE_27_4
 #define NAT_TILE_27_4 16, 8, 8
 #endif
-#ifndef NAT_PIPE_PASSES
-#define NAT_PIPE_PASSES 0  // later passes of a group: all LDS reads first, then the sums
-#endif
+
 
 namespace nat {
 
@@ -244,17 +242,6 @@ struct StructCfg {
     return m;
   }
   static constexpr int KI = (first_items() + NT - 1) / NT;
-  // register slots for a later pass
-  static constexpr int later_items() {
-    int m = 1;
-    for (int al = 1; al < Q; al++)
-      for (int k = 1; k < 3; k++) {
-        const PG g(D, P, TX, TY, TZ, Lattice<Q>::e[al][0], Lattice<Q>::e[al][1], (D == 3) ? Lattice<Q>::e[al][2] : 0, k);
-        if (k < g.np && g.items > m) m = g.items;
-      }
-    return m;
-  }
-  static constexpr int KL = (later_items() + NT - 1) / NT;
 };
 
 // SYM: the host found the 1-D tables identical on all axes and mirror symmetric,
