@@ -314,6 +314,8 @@ def run_serving(nat, reqs, precision, seed, load=0.8):
     srv = nat.Server(max_active=8, chunk_steps=50, use_graphs=True)
     try:
         closed, t_closed = submit_all(srv, [0.0] * len(reqs))
+        print(f"serving {precision}: closed loop of {len(reqs)} requests in {t_closed:.2f} s", file=sys.stderr,
+              flush=True)
         st = srv.stats()
     finally:
         srv.close()
@@ -324,6 +326,7 @@ def run_serving(nat, reqs, precision, seed, load=0.8):
     srv = nat.Server(max_active=8, chunk_steps=50, use_graphs=True)
     try:
         opened, t_open = submit_all(srv, list(arrivals))
+        print(f"serving {precision}: open loop in {t_open:.2f} s", file=sys.stderr, flush=True)
     finally:
         srv.close()
     lat = np.array([o["latency_ms"] for o in opened])
@@ -334,7 +337,8 @@ def run_serving(nat, reqs, precision, seed, load=0.8):
             "open_loop_glups": round(nodes / t_open / 1e9, 4),
             "latency_p50_ms": round(float(np.percentile(lat, 50)), 3),
             "latency_p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "node_updates": int(nodes), "server_busy_ms": round(st["busy_ms"], 1)}
+            "node_updates": int(nodes), "server_busy_ms": round(st["busy_ms"], 1),
+            "packed_chunks_closed_loop": int(st["packed_chunks"])}
 
 
 def main():

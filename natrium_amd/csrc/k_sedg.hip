@@ -26,6 +26,13 @@
 // faces come from other blocks). The last stage of a step can collide in
 // registers (CFDSolver::run order: stream, then collide).
 //
+// Walls (velocity-neq bounce-back on the faces of a non-periodic axis, assembled by
+// VelocityNeqBounceBack::assembleBoundary, VelocityNeqBounceBack.cpp:46-135): the other
+// side of a wall face is the opposite direction at the same node, and the wall velocity
+// enters as the system vector b of the RK5 right-hand side, with rho = 1:
+//   upwind (s_a G_a < 0): s_a G_a / w_a(i) * (f_i - f_opp,i) - 2 c w (e.u_wall) / cs2
+//   central:  1/2 c (f_i + f_opp,i) - c w (e.u_wall) / cs2,   c = s_a G_a / w_a(i).
+//
 // FP32 storage keeps f~ = f - w rho0 in FP32. L annihilates constants for the
 // upwind flux; for the central flux L(w rho0) = w rho0 sum_a s_a G_a / w_a(i) is
 // added back explicitly. All arithmetic is FP64.
@@ -74,7 +81,9 @@ struct SedgShape {
 
 __device__ __forceinline__ void sedg_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-template <int D, int Q, int P, typename T, bool COL>
+// WALLS: the mesh has a non-periodic axis (its own instantiation: the wall terms cost
+// registers that the periodic kernels keep for occupancy)
+template <int D, int Q, int P, typename T, bool COL, bool WALLS>
 __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T> s, CollideArgs<T> ca,
                                                                        int write_macro) {
   using L = Lattice<Q>;
@@ -96,6 +105,7 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
   double Ji[3][3];
   int64_t nbnode[3] = {0, 0, 0};
   double fac[3] = {0, 0, 0};  // s_a / w_a(i) for the face through this node along axis a, else 0
+  int wallf[3] = {-1, -1, -1};  // box face 2 a + side when that face is a wall
   {
     const int N0 = s.m.N[0], N1 = s.m.N[1];
     const int c[3] = {(int)(cell % N0), (int)((cell / N0) % N1), (D == 3) ? (int)(cell / ((int64_t)N0 * N1)) : 0};
@@ -114,9 +124,15 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
         const int up = (mi[a] == P) ? 1 : 0;
         int cn[3] = {c[0], c[1], c[2]};
         const int n = s.m.N[a];
-        cn[a] = up ? (c[a] + 1 == n ? 0 : c[a] + 1) : (c[a] == 0 ? n - 1 : c[a] - 1);
-        const int64_t nc = cn[0] + (int64_t)N0 * (cn[1] + (int64_t)N1 * cn[2]);
-        nbnode[a] = active ? nc * NL + l + (up ? -P : P) * lstr[a] : 0;
+        const bool edge = up ? (c[a] + 1 == n) : (c[a] == 0);
+        if (WALLS && edge && !s.m.periodic[a]) {
+          wallf[a] = 2 * a + up;
+          nbnode[a] = node;  // read in the opposite direction
+        } else {
+          cn[a] = up ? (c[a] + 1 == n ? 0 : c[a] + 1) : (c[a] == 0 ? n - 1 : c[a] - 1);
+          const int64_t nc = cn[0] + (int64_t)N0 * (cn[1] + (int64_t)N1 * cn[2]);
+          nbnode[a] = active ? nc * NL + l + (up ? -P : P) * lstr[a] : 0;
+        }
         fac[a] = (up ? 1.0 : -1.0) * s.inv_gw[mi[a]];
       }
     }
@@ -162,11 +178,13 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
     // than loading them after the volume term, which is kept there.
     constexpr bool early = sizeof(T) == 8;
     T fnb[3] = {T(0), T(0), T(0)};
+    constexpr int op = opposite<Q>(al);
+    const T* __restrict__ fop = s.fin + (int64_t)op * s.stride;
     auto load_faces = [&]() {
 #pragma unroll
       for (int a = 0; a < D; a++) {
         const double c = fac[a] * G[a];
-        if (active && (s.central ? fac[a] != 0.0 : c < 0.0)) fnb[a] = fin[nbnode[a]];
+        if (active && (s.central ? fac[a] != 0.0 : c < 0.0)) fnb[a] = (!WALLS || wallf[a] < 0 ? fin : fop)[nbnode[a]];
       }
     };
     if constexpr (early) load_faces();
@@ -187,14 +205,26 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
 #pragma unroll
     for (int a = 0; a < D; a++) {
       const double c = fac[a] * G[a];
+      // wall velocity term: c w (e.u_wall) / cs2 with e and cs2 scaled, cs2 = s^2 / 3
+      double cwu = 0.0;
+      if (WALLS && wallf[a] >= 0) {
+        const double* uw = s.m.wall_u[wallf[a]];
+        double eu = 0.0;
+        if (e0 != 0) eu += e0 * uw[0];
+        if (e1 != 0) eu += e1 * uw[1];
+        if (e2 != 0) eu += e2 * uw[2];
+        cwu = c * L::w[al] * (3.0 * eu / sc);
+      }
       if (s.central) {
         if (fac[a] != 0.0) {
           const double fn = (double)fnb[a];
           lf += 0.5 * c * (v + fn) + c * L::w[al] * s.rho0;
+          if constexpr (WALLS) lf -= cwu;
         }
       } else if (c < 0.0) {
         const double fn = (double)fnb[a];
         lf = fma(c, v - fn, lf);
+        if constexpr (WALLS) lf -= 2.0 * cwu;
       }
     }
     const double af = lf * dt;
@@ -219,10 +249,17 @@ template <int D, int Q, int P, typename T>
 static int launch_stage(nat_ctx* ctx, const SedgArgs<T>& s, const CollideArgs<T>& ca, int collide, int wm) {
   using S = SedgShape<D, P>;
   const unsigned grid = (unsigned)((s.ncell + S::CPB - 1) / S::CPB);
-  if (collide)
-    hipLaunchKernelGGL((k_sedg_stage<D, Q, P, T, true>), dim3(grid), dim3(S::NT), 0, ctx->stream, s, ca, wm);
-  else
-    hipLaunchKernelGGL((k_sedg_stage<D, Q, P, T, false>), dim3(grid), dim3(S::NT), 0, ctx->stream, s, ca, 0);
+  if (ctx->has_walls) {
+    if (collide)
+      hipLaunchKernelGGL((k_sedg_stage<D, Q, P, T, true, true>), dim3(grid), dim3(S::NT), 0, ctx->stream, s, ca, wm);
+    else
+      hipLaunchKernelGGL((k_sedg_stage<D, Q, P, T, false, true>), dim3(grid), dim3(S::NT), 0, ctx->stream, s, ca, 0);
+  } else {
+    if (collide)
+      hipLaunchKernelGGL((k_sedg_stage<D, Q, P, T, true, false>), dim3(grid), dim3(S::NT), 0, ctx->stream, s, ca, wm);
+    else
+      hipLaunchKernelGGL((k_sedg_stage<D, Q, P, T, false, false>), dim3(grid), dim3(S::NT), 0, ctx->stream, s, ca, 0);
+  }
   NAT_HIP(hipGetLastError());
   return 0;
 }
@@ -282,8 +319,10 @@ static int sedg_step_typed(nat_ctx* ctx, int collide_after, int write_macro) {
   s.m.D = ctx->D;
   for (int a = 0; a < 3; a++) {
     s.m.N[a] = a < ctx->D ? ctx->desc.ncells[a] : 1;
-    s.m.periodic[a] = 1;
+    s.m.periodic[a] = a < ctx->D ? (ctx->desc.periodic[a] ? 1 : 0) : 1;
   }
+  for (int f = 0; f < 6; f++)
+    for (int d = 0; d < 3; d++) s.m.wall_u[f][d] = d < ctx->D ? ctx->desc.wall_velocity[f][d] : 0.0;
   s.m.cartesian = 0;
   s.m.vert = ctx->d_vert;
   s.ncell = (int64_t)s.m.N[0] * s.m.N[1] * s.m.N[2];

@@ -643,9 +643,10 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
     if (d.ncells[a] < 1) return fail(ctx, NAT_ERR_ARG, "ncells must be >= 1");
     if (!d.periodic[a]) ctx->has_walls = true;
   }
-  if (ctx->has_walls) {
-    // velocity-neq bounce-back walls are built for SL streaming (SemiLagrangian.cpp:358-385)
-    if (d.scheme != NAT_SL) return fail(ctx, NAT_ERR_UNSUPPORTED, "walls are supported with SL streaming only");
+  if (ctx->has_walls && d.scheme == NAT_SL) {
+    // velocity-neq bounce-back walls of SL streaming (SemiLagrangian.cpp:358-385) are built in
+    // the departure walk of the general backend; SEDG applies them as a wall flux
+    // (VelocityNeqBounceBack.cpp:46-135) in its stage kernel
     if (d.backend == NAT_BACKEND_STRUCTURED)
       return fail(ctx, NAT_ERR_UNSUPPORTED, "the structured backend needs periodic boundaries on every axis");
     d.backend = NAT_BACKEND_GENERAL;

@@ -76,6 +76,8 @@ def lib():
         L.orc_op_src.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int8)]
         L.orc_op_bvec.restype = ctypes.c_int
         L.orc_op_bvec.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_op_opp.restype = ctypes.c_int
+        L.orc_op_opp.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.orc_collide_init.restype = ctypes.c_int
         L.orc_collide_init.argtypes = [ctypes.c_void_p, _dp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
@@ -218,6 +220,11 @@ class Oracle:
         if lib().orc_op_src(self.h, alpha, _p(src, ctypes.POINTER(ctypes.c_int8))):
             return src
         return None
+
+    def opposite_coupling(self, alpha):
+        """SEDG walls: diagonal of the operator block (alpha, opposite alpha), or None."""
+        d = np.zeros(self.n)
+        return d if lib().orc_op_opp(self.h, alpha, _p(d)) else None
 
     def boundary_values(self, alpha):
         """Values added to streamed f_alpha at wall hits (None: no hit)."""

@@ -148,7 +148,10 @@ int64_t orc_op_identity_rows(orc_solver* s) { return s->op.n_identity_rows; }
 void orc_set_walls(orc_solver* s, const double* wall_u) {
   for (int f = 0; f < 6; f++)
     for (int d = 0; d < 3; d++) s->mesh.wall_u[f][d] = wall_u[3 * f + d];
-  if (s->scheme == 0) s->feq.mesh = s->mesh;
+  if (s->scheme == 0)
+    s->feq.mesh = s->mesh;
+  else
+    s->dg.mesh = s->mesh;
 }
 
 // Boundary hit list counts of the last assembly (SemiLagrangianBoundaryHandler::n_hits, n_cells)
@@ -169,6 +172,13 @@ int orc_op_bvec(orc_solver* s, int alpha, double* b) {
   const std::vector<double>& v = s->op.bvec[alpha - 1];
   if (v.empty()) return 0;
   std::memcpy(b, v.data(), sizeof(double) * v.size());
+  return 1;
+}
+// SEDG walls: diagonal of the block (alpha, opposite alpha) (returns 0 without walls)
+int orc_op_opp(orc_solver* s, int alpha, double* d) {
+  if (s->op.opp_diag.empty()) return 0;
+  const std::vector<double>& v = s->op.opp_diag[alpha - 1];
+  std::memcpy(d, v.data(), sizeof(double) * v.size());
   return 1;
 }
 void orc_op_get(orc_solver* s, int alpha, int64_t* rowptr, int64_t* col, double* val) {

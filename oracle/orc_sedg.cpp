@@ -74,6 +74,19 @@ void assemble_sedg(const DGSpace& V, const Stencil& st, bool central_flux, SLOpe
     for (int a = 0; a < D; a++) nf += (mi[a] == 0 || mi[a] == V.p) ? 1 : 0;
     rowlen[i] = nl + nf;
   }
+  // walls on non-periodic axes: the blocks (alpha, opposite alpha) and the system vector
+  bool walls = false;
+  for (int a = 0; a < D; a++) walls = walls || !mesh.periodic[a];
+  op.opp_block.assign(st.Q - 1, -1);
+  op.opp_diag.assign(walls ? st.Q - 1 : 0, std::vector<double>());
+  op.bvec.assign(st.Q - 1, std::vector<double>());
+  for (int alpha = 1; alpha < st.Q; alpha++) {
+    op.opp_block[alpha - 1] = st.opp[alpha] - 1;
+    if (walls) {
+      op.opp_diag[alpha - 1].assign(n, 0.0);
+      op.bvec[alpha - 1].assign(n, 0.0);
+    }
+  }
   for (int alpha = 1; alpha < st.Q; alpha++) {
     CSR& A = op.block[alpha - 1];
     A.rowptr.assign(n + 1, 0);
@@ -117,7 +130,8 @@ void assemble_sedg(const DGSpace& V, const Stencil& st, bool central_flux, SLOpe
     // face geometry per (face, q): neighbour DoF, local DoF, JxW*normal
     struct FaceQ {
       int li;
-      int64_t gn;
+      int64_t gn;  // neighbour's coincident node, -1 on a wall
+      int face;    // box face 2*axis+side (walls)
       double nrm[3];
       double jxw;
     };
@@ -126,8 +140,7 @@ void assemble_sedg(const DGSpace& V, const Stencil& st, bool central_flux, SLOpe
     for (int f = 0; f < 2 * D; f++) {
       const int a = f / 2, side = f % 2;
       bool wrapped;
-      int nb = neighbor_cell(mesh, c, f, &wrapped);
-      if (nb < 0) throw std::runtime_error("non-periodic boundary: not supported by the SEDG oracle");
+      int nb = neighbor_cell(mesh, c, f, &wrapped);  // -1: a wall of a non-periodic axis
       const int tb = (a == 0) ? 1 : 0;
       const int tc = (a == 2) ? 1 : 2;
       const int nq = D == 3 ? np * np : np;
@@ -144,7 +157,8 @@ void assemble_sedg(const DGSpace& V, const Stencil& st, bool central_flux, SLOpe
         inv_and_det(J, D, Ji, &det);
         FaceQ F;
         F.li = lidx(m[0], m[1], m[2]);
-        F.gn = (int64_t)nb * nl + lidx(mn[0], mn[1], mn[2]);
+        F.gn = nb < 0 ? -1 : (int64_t)nb * nl + lidx(mn[0], mn[1], mn[2]);
+        F.face = f;
         double len = 0;
         const double sgn = side ? 1.0 : -1.0;
         for (int k = 0; k < 3; k++) F.nrm[k] = 0;
@@ -183,6 +197,28 @@ void assemble_sedg(const DGSpace& V, const Stencil& st, bool central_flux, SLOpe
           for (int k = 0; k < D; k++) exn += F.nrm[k] * e[k];
           prefactor *= exn;
           double cell_entry = 0.0, neighbor_entry = 0.0;
+          if (F.gn < 0) {
+            // velocity-neq bounce-back wall (VelocityNeqBounceBack.cpp:46-135): density 1, the
+            // wall velocity of the face; the other side of the face is the opposite direction
+            // at the same node, and the wall velocity enters the system vector
+            double exu = 0.0;
+            for (int k = 0; k < D; k++) exu += mesh.wall_u[F.face][k] * e[k];
+            const double density = 1.0;
+            double opp_entry = 0.0, vec_entry = 0.0;
+            if (central_flux) {
+              cell_entry = 0.5 * prefactor;
+              opp_entry = 0.5 * prefactor;
+              vec_entry = -prefactor * st.w[alpha] * density * exu / st.cs2;
+            } else if (exn < 0) {
+              cell_entry = prefactor;
+              opp_entry = -prefactor;
+              vec_entry = -2 * prefactor * st.w[alpha] * density * exu / st.cs2;
+            }
+            row[i].second += cell_entry * invM[i];
+            op.opp_diag[alpha - 1][gi] += opp_entry * invM[i];
+            op.bvec[alpha - 1][gi] += vec_entry * invM[i];
+            continue;
+          }
           if (central_flux) {
             cell_entry = 0.5 * prefactor;
             neighbor_entry = 0.5 * prefactor;
@@ -202,26 +238,40 @@ void assemble_sedg(const DGSpace& V, const Stencil& st, bool central_flux, SLOpe
           A.val[o] = kv.second;
           o++;
         }
+        for (; o < A.rowptr[gi + 1]; o++) {  // slots of wall faces (no neighbour entry)
+          A.col[o] = gi;
+          A.val[o] = 0.0;
+        }
       }
     }
   }
 }
 
 void rk5_step(const SLOperator& op, double dt, double* const* f, std::vector<double>& work) {
-  // RungeKutta5LowStorage.cpp:56-105, operation by operation
+  // RungeKutta5LowStorage.cpp:56-105, operation by operation; the product is the block
+  // matrix's (diagonal blocks, plus the (alpha, opposite alpha) diagonals of wall faces)
   const int nb = op.Q - 1;
   const int64_t n = op.n;
   work.resize((size_t)2 * nb * n);
   double* Af = work.data();
   double* df = work.data() + (size_t)nb * n;
+  const bool walls = !op.opp_diag.empty();
   for (int s = 0; s < 5; s++) {
     for (int b = 0; b < nb; b++) csr_vmult(op.block[b], f[b], Af + (size_t)b * n, n);
+    if (walls) {  // blocks (alpha, opposite alpha) of the wall faces: diagonal
+#pragma omp parallel for schedule(static)
+      for (int64_t t = 0; t < (int64_t)nb * n; t++) {
+        const int b = (int)(t / n);
+        const int64_t i = t % n;
+        Af[t] += op.opp_diag[b][i] * f[op.opp_block[b]][i];
+      }
+    }
 #pragma omp parallel for schedule(static)
     for (int64_t t = 0; t < (int64_t)nb * n; t++) {
       const int b = (int)(t / n);
       const int64_t i = t % n;
       double af = Af[t];
-      af += 0.0;  // systemVector is zero for periodic problems
+      if (walls) af += op.bvec[b][i];  // m_Af += systemVector (zero without walls)
       af *= dt;
       double d;
       if (s == 0) {

@@ -7,6 +7,8 @@
 //   advection/SEDGMinLee.cpp:361-412  face loop incl. periodic faces
 //   advection/SEDGMinLee.cpp:414-458  -(e . D) M^-1 cell block
 //   advection/SEDGMinLee.cpp:461-536  upwind / central face terms
+//   advection/SEDGMinLee.cpp:361-412, boundaries/VelocityNeqBounceBack.cpp:46-135
+//                                      velocity-neq bounce-back wall faces
 //   timeintegration/RungeKutta5LowStorage.h:29-46, .cpp:56-105
 #pragma once
 #include <vector>

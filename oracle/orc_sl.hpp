@@ -58,6 +58,11 @@ struct SLOperator {
   // hit-list counts of the boundary handler (hits, distinct cells holding hits).
   std::vector<std::vector<double>> bvec;
   int64_t n_hits = 0, n_hit_cells = 0;
+  // SEDG with velocity-neq bounce-back walls (VelocityNeqBounceBack::assembleBoundary): the
+  // diagonal of the block (alpha, opposite alpha) per destination block (empty without walls),
+  // the block index of the opposite direction, and bvec above as the system vector b
+  std::vector<std::vector<double>> opp_diag;
+  std::vector<int> opp_block;
 };
 
 struct WallHit {

@@ -151,17 +151,20 @@ int main(int argc, char** argv) {
     std::printf("rk5 t=%.17g dt=%.17g\n", t, rk.getTimeStepSize());
   }
 
-  // a descriptor the build does not support: SEDG with walls
+  // a descriptor the build does not support: compressible populations with SEDG streaming
   {
-    nat_desc d = tgv_desc(v, N, p, NAT_SEDG);
-    d.periodic[0] = 0;
+    nat_desc d = tgv_desc(v, N, 2, NAT_SEDG);
+    d.stencil = NAT_D2Q25H;
+    d.compressible = 1;
+    d.equilibrium = NAT_EQ_QUARTIC;
+    d.gamma = 1.4;
     try {
-      CFDSolver<2> s(d, SolverConfiguration{});
-      std::printf("sedg walls: no exception\n");
+      CompressibleCFDSolver<2> s(d, SolverConfiguration{});
+      std::printf("compressible sedg: no exception\n");
     } catch (const NotImplementedException&) {
-      std::printf("sedg walls: NotImplementedException\n");
+      std::printf("compressible sedg: NotImplementedException\n");
     } catch (const NATriuMException& e) {
-      std::printf("sedg walls: NATriuMException code %d\n", e.code());
+      std::printf("compressible sedg: NATriuMException code %d\n", e.code());
     }
   }
   std::printf("host_layer_check done\n");

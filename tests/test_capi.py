@@ -58,16 +58,16 @@ def test_struct_layouts_match_header(tmp_path, cname, pyname):
         assert getattr(cls, f).offset == off, f
 
 
-def test_walls_rejected_before_device_use():
-    """Descriptor checks run before any device call: walls need SL streaming and the
-    general backend."""
+def test_unsupported_descriptors_rejected_before_device_use():
+    """Descriptor checks run before any device call: SL walls need the general backend, and
+    compressible populations use SL streaming."""
     import numpy as np
     import pytest
 
     from natrium_amd.problems import box_vertices
 
     v = box_vertices(2, [3, 3], [0, 0], [1, 1])
-    with pytest.raises(nat.NatError, match="SL streaming"):
-        nat.Solver("D2Q9", 2, [3, 3], v, periodic=[1, 0], scheme="SEDG")
     with pytest.raises(nat.NatError, match="periodic boundaries on every axis"):
         nat.Solver("D2Q9", 2, [3, 3], np.asarray(v), periodic=[1, 0], backend="structured")
+    with pytest.raises(nat.NatError, match="SL streaming"):
+        nat.Solver("D2Q25H", 2, [3, 3], v, compressible=True, scheme="SEDG")

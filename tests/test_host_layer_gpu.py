@@ -82,7 +82,7 @@ def test_semilagrangian_stream_and_select_collision_match_oracle(run):
     assert abs(dt - o.dt) < 1e-18
     assert "same-vector stream: SemiLagrangianException" in stdout
     assert "zero density: DensityZeroException" in stdout
-    assert "sedg walls: NotImplementedException" in stdout
+    assert "compressible sedg: NotImplementedException" in stdout
     assert "host_layer_check done" in stdout
 
 

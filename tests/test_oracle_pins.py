@@ -16,6 +16,15 @@ mathematics fixes (CPU only).
   first-order perturbation g = g_eq + eps w e_x has g* = (cs2 / T) g_neq, so its first
   moment relaxes by (1 - omega_v) + (omega_v - omega_Pr) cs2 / T (the reference's g* carries
   no 1/cs2, so the g flux does not relax at the Prandtl rate; kept for parity).
+- SEDG velocity-neq bounce-back walls (VelocityNeqBounceBack.cpp:46-135): at a wall face a
+  direction entering the domain (e.n < 0) reads the opposite direction at the same node,
+  +JxW_f (e.n)/M on the diagonal, -JxW_f (e.n)/M on the block (alpha, opposite alpha), and
+  b = -2 JxW_f (e.n) w rho (e.u_wall) / cs2 / M (rho = 1). Two facts fix all three:
+  the mass sum_alpha JxW^T (L_alpha f + C_alpha f_opp + b_alpha) vanishes for any f with
+  walls moving along themselves (the flux of every incoming direction is the outgoing flux
+  of its opposite), and the equilibrium at the wall velocity, f_eq(1, u_wall), is stationary
+  (f_alpha - f_opp = 2 w e.u / cs2 is exactly what b removes), which fails for a wrong factor
+  of b or a wrong sign of the opposite entry.
 - The initialisation collision (CollisionOperator.h:79-91, init == true): the given
   velocity drives the relaxation, j' = (1 - omega) j + omega rho u_given, mass kept,
   f_eq(rho, u_given) a fixed point, and with u_given = the computed velocity it equals the
@@ -262,3 +271,80 @@ def test_collide_init_relaxes_toward_the_given_velocity():
     o.set_f(feq)
     o.collide_init()
     np.testing.assert_allclose(o.get_f(), feq, rtol=0, atol=1e-15)
+
+
+def _sedg_wall_oracle(stencil, p, N, periodic, wall_velocity, seed, central=False):
+    D = len(N)
+    L = 2 * math.pi
+    v = perturbed_vertices(D, N, [0] * D, [L] * D, amplitude=0.1, seed=seed)
+    o = O.Oracle(stencil, p, N, v, cartesian=False, periodic=periodic, scaling=1.7, viscosity=1.0,
+                 scheme="SEDG_CENTRAL" if central else "SEDG", wall_velocity=wall_velocity)
+    o.assemble()
+    return o, v
+
+
+def _wall_rhs(o, f):
+    """(L f + C f_opp + b) per direction alpha = 1..Q-1, f of shape (Q, n)."""
+    _, _, opp, _ = O.stencil(["D2Q9", "D3Q19", "D3Q27"][[9, 19, 27].index(o.Q)], 1.7)
+    out = np.zeros((o.Q, o.n))
+    for a in range(1, o.Q):
+        C = o.opposite_coupling(a)
+        b = o.boundary_values(a)
+        assert C is not None and b is not None
+        out[a] = o.operator(a) @ f[a] + C * f[opp[a]] + b
+    return out, opp
+
+
+# walls on y (2D) or z (3D), moving along themselves
+WALL_CASES = [("D2Q9", 3, [3, 4], [1, 0], {2: [0.0, 0.0], 3: [0.0, 0.0]}, 5),
+              ("D2Q9", 4, [4, 3], [1, 0], {2: [-0.2, 0.0], 3: [0.3, 0.0]}, 7),
+              ("D3Q19", 2, [2, 3, 3], [1, 1, 0], {4: [0.1, -0.2, 0.0], 5: [0.25, 0.05, 0.0]}, 11),
+              ("D3Q27", 2, [3, 2, 2], [1, 0, 1], {2: [0.1, 0.0, -0.1], 3: [-0.3, 0.0, 0.2]}, 13)]
+
+
+@pytest.mark.parametrize("stencil,p,N,periodic,wu,seed", WALL_CASES)
+def test_sedg_wall_flux_conserves_mass(stencil, p, N, periodic, wu, seed):
+    o, v = _sedg_wall_oracle(stencil, p, N, periodic, wu, seed)
+    jxw = _dg_jxw(len(N), N, p, v)
+    f = np.random.default_rng(seed).standard_normal((o.Q, o.n))
+    r, opp = _wall_rhs(o, f)
+    tot = sum(jxw @ r[a] for a in range(1, o.Q))
+    scale = sum(jxw @ np.abs(r[a]) for a in range(1, o.Q))
+    assert abs(tot) < 1e-13 * scale, abs(tot) / scale
+    # a wrong sign on the opposite-direction entries breaks it
+    bad = sum(jxw @ (r[a] - 2 * o.opposite_coupling(a) * f[opp[a]]) for a in range(1, o.Q))
+    assert abs(bad) > 1e-6 * scale
+
+
+@pytest.mark.parametrize("stencil,p,N,periodic,wu,seed", WALL_CASES)
+def test_sedg_wall_equilibrium_at_wall_velocity_is_stationary(stencil, p, N, periodic, wu, seed):
+    """f = f_eq(rho = 1, u_wall) with both walls at one tangential velocity: L f + C f_opp + b = 0."""
+    u = np.array(list(wu.values())[0])
+    for face in wu:  # one velocity on both walls
+        wu[face] = list(u)
+    o, _ = _sedg_wall_oracle(stencil, p, N, periodic, wu, seed)
+    e, w, opp, cs2 = O.stencil(stencil, 1.7)
+    eu = e @ u[:len(N)]
+    feq = w * (1 + eu / cs2 + eu ** 2 / (2 * cs2 ** 2) - (u @ u) / (2 * cs2))
+    f = np.repeat(feq[:, None], o.n, axis=1)
+    r, _ = _wall_rhs(o, f)
+    scale = max(np.abs(o.operator(a)).max() for a in range(1, o.Q)) * np.abs(f).max()
+    assert np.abs(r[1:]).max() < 1e-13 * scale, np.abs(r[1:]).max() / scale
+    if np.abs(u).max() > 0:
+        # half the system vector (or none) leaves a residual at the wall nodes
+        half = max(np.abs(r[a] - 0.5 * o.boundary_values(a)).max() for a in range(1, o.Q))
+        assert half > 1e-6 * scale
+
+
+def test_sedg_wall_rk5_step_keeps_wall_equilibrium():
+    """One RK5 step of the oracle (orc_stream, RungeKutta5LowStorage.cpp:56-105 with the
+    system vector) from f_eq(1, u_wall) leaves it unchanged."""
+    u = [0.2, 0.0]
+    o, _ = _sedg_wall_oracle("D2Q9", 3, [3, 4], [1, 0], {2: u, 3: u}, 3)
+    e, w, _, cs2 = O.stencil("D2Q9", 1.7)
+    eu = e @ np.array(u)
+    feq = w * (1 + eu / cs2 + eu ** 2 / (2 * cs2 ** 2) - (np.array(u) @ np.array(u)) / (2 * cs2))
+    f0 = np.repeat(feq[:, None], o.n, axis=1)
+    o.set_f(f0)
+    o.stream()
+    assert np.abs(o.get_f() - f0).max() < 1e-14
