@@ -16,6 +16,8 @@
 #include <sys/stat.h>
 #include <vector>
 
+#include <zlib.h>
+
 #include "nat_internal.hpp"
 
 using namespace nat;
@@ -289,6 +291,33 @@ int nat_write_stats(nat_ctx* ctx, const char* path, int64_t iteration, int trunc
   return out ? 0 : fail(ctx, NAT_ERR_ARG, std::string("write failed: ") + path);
 }
 
+// base64 (RFC 4648) of a byte string
+static std::string base64(const unsigned char* p, size_t n) {
+  static const char* tab = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+  std::string out;
+  out.reserve((n + 2) / 3 * 4);
+  for (size_t i = 0; i < n; i += 3) {
+    const uint32_t b0 = p[i], b1 = i + 1 < n ? p[i + 1] : 0, b2 = i + 2 < n ? p[i + 2] : 0;
+    const uint32_t v = (b0 << 16) | (b1 << 8) | b2;
+    out += tab[(v >> 18) & 63];
+    out += tab[(v >> 12) & 63];
+    out += i + 1 < n ? tab[(v >> 6) & 63] : '=';
+    out += i + 2 < n ? tab[v & 63] : '=';
+  }
+  return out;
+}
+
+// One VTK zlib block: base64 of the header (blocks = 1, block size, last block size,
+// compressed size, UInt32 each), then base64 of the compressed bytes.
+static bool zlib_base64(const void* data, size_t bytes, std::string& out) {
+  uLongf clen = compressBound((uLong)bytes);
+  std::vector<unsigned char> buf(clen);
+  if (compress2(buf.data(), &clen, (const Bytef*)data, (uLong)bytes, Z_BEST_COMPRESSION) != Z_OK) return false;
+  const uint32_t hdr[4] = {1u, (uint32_t)bytes, (uint32_t)bytes, (uint32_t)clen};
+  out = base64((const unsigned char*)hdr, sizeof(hdr)) + base64(buf.data(), clen);
+  return true;
+}
+
 // Patches as DataOut::build_patches(n) lays them out: per cell (n+1)^D points on a uniform
 // unit-cell grid (n = p + 1, or 1 for p = 1), values interpolated from the cell's DoFs,
 // positions through the cell map; n^D linear sub-cells per patch. Point data rho, ux, uy
@@ -351,46 +380,61 @@ int nat_write_vtu(nat_ctx* ctx, const char* dir, int64_t iteration) {
   std::ostringstream name;
   name << "t_0." << iteration << ".vtu";
   const std::string path = std::string(dir) + "/" + name.str();
-  std::ofstream o(path);
+  std::ofstream o(path, std::ios::binary);
   if (!o) return fail(ctx, NAT_ERR_ARG, "cannot open " + path);
-  o.precision(17);
   const int64_t npts = ncell * pts_per, ncl = ncell * cells_per;
   const char* names[5] = {"rho", "ux", "uy", "uz", "T"};
-  o << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n"
-    << "<UnstructuredGrid>\n<Piece NumberOfPoints=\"" << npts << "\" NumberOfCells=\"" << ncl << "\">\n"
-    << "<Points>\n<DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n";
-  for (int64_t i = 0; i < npts; i++) o << xyz[3 * i] << " " << xyz[3 * i + 1] << " " << xyz[3 * i + 2] << "\n";
-  o << "</DataArray>\n</Points>\n<Cells>\n<DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+  // The arrays go out as deal.II's write_vtu writes them with zlib: format="binary", each
+  // array one zlib block, base64 of the 4 x UInt32 block header, then base64 of the block.
+  std::string err;
+  auto arr = [&](const char* type, const char* attrs, const void* data, size_t bytes) {
+    std::string enc;
+    if (!zlib_base64(data, bytes, enc)) err = "zlib compression failed";
+    o << "<DataArray type=\"" << type << "\" " << attrs << " format=\"binary\">\n" << enc << "\n</DataArray>\n";
+  };
+  o << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" "
+       "compressor=\"vtkZLibDataCompressor\">\n"
+    << "<UnstructuredGrid>\n<Piece NumberOfPoints=\"" << npts << "\" NumberOfCells=\"" << ncl << "\">\n<Points>\n";
+  arr("Float64", "NumberOfComponents=\"3\"", xyz.data(), xyz.size() * sizeof(double));
+  o << "</Points>\n<Cells>\n";
+  std::vector<int32_t> conn;
+  conn.reserve((size_t)ncl * (D == 3 ? 8 : 4));
   for (int64_t cell = 0; cell < ncell; cell++)
     for (int s2 = 0; s2 < (D == 3 ? sub : 1); s2++)
       for (int s1 = 0; s1 < sub; s1++)
         for (int s0 = 0; s0 < sub; s0++) {
-          auto P = [&](int a, int b, int c) { return cell * pts_per + (s0 + a) + pp * ((s1 + b) + (int64_t)pp * (s2 + c)); };
-          o << P(0, 0, 0) << " " << P(1, 0, 0) << " " << P(1, 1, 0) << " " << P(0, 1, 0);
-          if (D == 3) o << " " << P(0, 0, 1) << " " << P(1, 0, 1) << " " << P(1, 1, 1) << " " << P(0, 1, 1);
-          o << "\n";
+          auto P = [&](int a, int b, int c) {
+            return (int32_t)(cell * pts_per + (s0 + a) + pp * ((s1 + b) + (int64_t)pp * (s2 + c)));
+          };
+          for (int32_t v : {P(0, 0, 0), P(1, 0, 0), P(1, 1, 0), P(0, 1, 0)}) conn.push_back(v);
+          if (D == 3)
+            for (int32_t v : {P(0, 0, 1), P(1, 0, 1), P(1, 1, 1), P(0, 1, 1)}) conn.push_back(v);
         }
-  o << "</DataArray>\n<DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
+  if (npts > INT32_MAX) return fail(ctx, NAT_ERR_UNSUPPORTED, "VTU output: more than 2^31 patch points");
+  arr("Int32", "Name=\"connectivity\"", conn.data(), conn.size() * sizeof(int32_t));
   const int nvc = D == 3 ? 8 : 4;
-  for (int64_t i = 1; i <= ncl; i++) o << i * nvc << "\n";
-  o << "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n";
-  for (int64_t i = 0; i < ncl; i++) o << (D == 3 ? 12 : 9) << "\n";
-  o << "</DataArray>\n</Cells>\n<PointData Scalars=\"rho\">\n";
+  std::vector<int32_t> offs((size_t)ncl);
+  for (int64_t i = 0; i < ncl; i++) offs[(size_t)i] = (int32_t)((i + 1) * nvc);
+  arr("Int32", "Name=\"offsets\"", offs.data(), offs.size() * sizeof(int32_t));
+  std::vector<uint8_t> types((size_t)ncl, (uint8_t)(D == 3 ? 12 : 9));
+  arr("UInt8", "Name=\"types\"", types.data(), types.size());
+  o << "</Cells>\n<PointData Scalars=\"rho\">\n";
   for (int fi = 0; fi < nfields; fi++) {
-    const char* nm = fi == 0 ? names[0] : (fi <= D ? names[fi] : names[4]);
-    o << "<DataArray type=\"Float64\" Name=\"" << nm << "\" format=\"ascii\">\n";
-    for (int64_t i = 0; i < npts; i++) o << val[fi][i] << "\n";
-    o << "</DataArray>\n";
+    const std::string nm = std::string("Name=\"") + (fi == 0 ? names[0] : (fi <= D ? names[fi] : names[4])) + "\"";
+    arr("Float64", nm.c_str(), val[fi].data(), val[fi].size() * sizeof(double));
   }
-  o << "</PointData>\n<CellData>\n<DataArray type=\"Float64\" Name=\"subdomain\" format=\"ascii\">\n";
-  for (int64_t i = 0; i < ncl; i++) o << "0\n";
-  o << "</DataArray>\n</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n";
+  o << "</PointData>\n<CellData>\n";
+  std::vector<double> subdomain((size_t)ncl, 0.0);
+  arr("Float64", "Name=\"subdomain\"", subdomain.data(), subdomain.size() * sizeof(double));
+  o << "</CellData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n";
+  if (!err.empty()) return fail(ctx, NAT_ERR_ARG, err);
   if (!o) return fail(ctx, NAT_ERR_ARG, "write failed: " + path);
   // master record listing the single piece
   std::ostringstream pname;
   pname << std::string(dir) << "/t_0." << iteration << ".pvtu";
   std::ofstream pv(pname.str());
-  pv << "<?xml version=\"1.0\"?>\n<VTKFile type=\"PUnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n"
+  pv << "<?xml version=\"1.0\"?>\n<VTKFile type=\"PUnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" "
+        "compressor=\"vtkZLibDataCompressor\">\n"
      << "<PUnstructuredGrid GhostLevel=\"0\">\n<PPointData Scalars=\"rho\">\n";
   for (int fi = 0; fi < nfields; fi++)
     pv << "<PDataArray type=\"Float64\" Name=\"" << (fi == 0 ? names[0] : (fi <= D ? names[fi] : names[4]))

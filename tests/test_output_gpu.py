@@ -99,6 +99,23 @@ def test_stats_residuals_and_table_file(tmp_path):
     assert abs(st["res_rho"] - cols[9]) <= 1e-5 * cols[9]
 
 
+def _vtk_array(el):
+    """Decode a DataArray written as deal.II's write_vtu writes it with zlib (format="binary",
+    compressor vtkZLibDataCompressor): base64 of the UInt32 block header (blocks, block size,
+    last block size, compressed sizes), then base64 of the zlib blocks."""
+    import base64
+    import zlib
+
+    assert el.get("format") == "binary"
+    text = el.text.strip()
+    hdr = np.frombuffer(base64.b64decode(text[:24]), dtype="<u4")
+    assert hdr[0] == 1 and hdr[1] == hdr[2]
+    raw = zlib.decompress(base64.b64decode(text[24:]))
+    assert len(raw) == hdr[1]
+    dt = {"Float64": "<f8", "Int32": "<i4", "UInt8": "u1"}[el.get("type")]
+    return np.frombuffer(raw, dtype=dt)
+
+
 @pytest.mark.parametrize("D,p", [(2, 1), (2, 3), (3, 2)])
 def test_vtu_patches_reproduce_polynomial_fields(tmp_path, D, p):
     N = [3, 2, 2][:D]
@@ -128,8 +145,16 @@ def test_vtu_patches_reproduce_polynomial_fields(tmp_path, D, p):
     npts = int(piece.get("NumberOfPoints"))
     assert npts == ncell * (sub + 1) ** D
     assert int(piece.get("NumberOfCells")) == ncell * sub ** D
-    pts = np.array(piece.find("Points/DataArray").text.split(), dtype=float).reshape(-1, 3)
-    arrays = {a.get("Name"): np.array(a.text.split(), dtype=float) for a in piece.findall("PointData/DataArray")}
+    assert root.get("compressor") == "vtkZLibDataCompressor"
+    pts = _vtk_array(piece.find("Points/DataArray")).reshape(-1, 3)
+    arrays = {a.get("Name"): _vtk_array(a) for a in piece.findall("PointData/DataArray")}
+    conn = _vtk_array(piece.find("Cells/DataArray[@Name='connectivity']"))
+    offs = _vtk_array(piece.find("Cells/DataArray[@Name='offsets']"))
+    types = _vtk_array(piece.find("Cells/DataArray[@Name='types']"))
+    nv = 4 if D == 2 else 8
+    assert conn.size == ncell * sub ** D * nv and conn.min() == 0 and conn.max() == npts - 1
+    np.testing.assert_array_equal(offs, nv * np.arange(1, ncell * sub ** D + 1))
+    assert (types == (9 if D == 2 else 12)).all()
     assert sorted(arrays) == sorted(["rho", "ux", "uy"] + (["uz"] if D == 3 else []))
     np.testing.assert_allclose(arrays["rho"], rho_f(pts), rtol=0, atol=1e-12)
     np.testing.assert_allclose(arrays["ux"], ux_f(pts), rtol=0, atol=1e-12)
