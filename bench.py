@@ -225,7 +225,7 @@ def extra_configs(nat, steps_sl=100, steps_err=500, cpu=False):
         del o
     mk_dg = lambda prec: nat.Solver("D3Q19", 3, [24] * 3, v3, cartesian=False, scaling=s, viscosity=0.01, cfl=0.4,
                                     precision=prec, scheme="SEDG")
-    out["c3_sedg_rk5"] = run(mk_dg, init3, max(steps_sl // 5, 10), steps_err // 5)
+    out["c3_sedg_rk5"] = run(mk_dg, init3, max(steps_sl // 5, 10), steps_err)
 
     v4 = box_vertices(2, [128, 128], [0, 0], [1, 1])
 

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "nat_collide.hpp"
+#include "nat_fastdiv.hpp"
 #include "nat_geom.hpp"
 
 namespace nat {
@@ -69,6 +70,7 @@ struct SumArgs {
   MeshDev m;
   int p;
   int layout;
+  FastDivU div_nl, div_n0, div_n1;  // (cell, node) index -> cell, node; cell -> coordinates
   int nn[3];        // FE_Q lattice nodes (LAYOUT_FEQ) or unique nodes (LAYOUT_UNIQUE) per axis
   int64_t mstride;  // stride of u components
   double gll[MAXP + 1];
@@ -98,10 +100,14 @@ __global__ __launch_bounds__(SUM_BS) void k_sums(SumArgs s, const T* __restrict_
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v[5] = {0, 0, 0, 0, 0};
   if (t < ncell * nl) {
-    const int64_t cell = t / nl;
-    const int l = (int)(t - cell * nl);
-    int c[3] = {(int)(cell % s.m.N[0]), (int)((cell / s.m.N[0]) % s.m.N[1]),
-                s.m.D == 3 ? (int)(cell / ((int64_t)s.m.N[0] * s.m.N[1])) : 0};
+    // 32-bit index arithmetic by multiply-high divisions (the host keeps ncell * nl < 2^31):
+    // 64-bit divisions by run-time values cost more than the sum itself
+    const uint32_t tt = (uint32_t)t;
+    const uint32_t cell = s.div_nl.div(tt);
+    const int l = (int)(tt - cell * (uint32_t)nl);
+    const uint32_t q0 = s.div_n0.div(cell), q1 = s.div_n1.div(q0);
+    int c[3] = {(int)(cell - q0 * (uint32_t)s.m.N[0]), (int)(q0 - q1 * (uint32_t)s.m.N[1]),
+                s.m.D == 3 ? (int)q1 : 0};
     int mi[3] = {l % np, (l / np) % np, s.m.D == 3 ? l / (np * np) : 0};
     double xi[3] = {s.gll[mi[0]], s.gll[mi[1]], s.m.D == 3 ? s.gll[mi[2]] : 0.0};
     double J[3][3];
@@ -253,7 +259,11 @@ static nat::SumArgs sum_args(const nat_ctx* ctx, int layout, int64_t* grid) {
   s.mstride = ctx->n_store;
   gll_points(ctx->p, s.gll, s.gllw);
   const int np = ctx->p + 1;
-  const int64_t total = (int64_t)s.m.N[0] * s.m.N[1] * s.m.N[2] * (ctx->D == 3 ? np * np * np : np * np);
+  const int nl = ctx->D == 3 ? np * np * np : np * np;
+  const int64_t total = (int64_t)s.m.N[0] * s.m.N[1] * s.m.N[2] * nl;
+  s.div_nl.set((uint32_t)nl);
+  s.div_n0.set((uint32_t)s.m.N[0]);
+  s.div_n1.set((uint32_t)s.m.N[1]);
   *grid = (total + SUM_BS - 1) / SUM_BS;
   return s;
 }
@@ -261,6 +271,10 @@ static nat::SumArgs sum_args(const nat_ctx* ctx, int layout, int64_t* grid) {
 // Both passes on the context's stream; partial holds 5 doubles per pass-1 block.
 static int launch_sums_into(nat_ctx* ctx, const nat::SumArgs& s, int64_t grid, double* partial) {
   using namespace nat;
+  if (grid * SUM_BS > INT32_MAX) {
+    ctx->err = "conservation sums: more than 2^31 (cell, node) pairs";
+    return NAT_ERR_UNSUPPORTED;
+  }
   if (ctx->fp32)
     hipLaunchKernelGGL((k_sums<float>), dim3((unsigned)grid), dim3(SUM_BS), 0, ctx->stream, s,
                        (const float*)ctx->rho, (const float*)ctx->u, partial);

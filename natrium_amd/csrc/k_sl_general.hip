@@ -13,6 +13,7 @@
 #include "nat_collide.hpp"
 #include "nat_fastdiv.hpp"
 
+
 namespace nat {
 
 // Workgroup index regrouped so that each XCD (workgroup i is dispatched to XCD i % 8)
@@ -119,13 +120,41 @@ __device__ __forceinline__ void load_row(__amdgpu_buffer_rsrc_t r, int off, doub
 // rows take the term-by-term path with per-lane source addresses. With FP32
 // storage the shift w rho0 passes through, since opposite directions share w.
 //
-// Returns the streamed population of direction al at DoF i.
+// The departure record of one (direction, DoF): FP64 storage the cell and FP64 unit
+// coordinates, FP32 storage the packed cell/anchor code and fixed-point offsets.
+template <int D, typename T>
+struct DepRec {
+  int c;
+  double xi[D];
+};
+template <int D>
+struct DepRec<D, float> {
+  int c;
+  int off[D];
+};
+
+template <int D, typename T>
+__device__ __forceinline__ DepRec<D, T> load_rec(const GeneralArgs<T>& a, int64_t r) {
+  DepRec<D, T> rec;
+  if constexpr (sizeof(T) == 8) {
+    rec.c = a.cell[r];
+#pragma unroll
+    for (int d = 0; d < D; d++) rec.xi[d] = a.xi[r * D + d];
+  } else {
+    rec.c = a.code[r];
+#pragma unroll
+    for (int d = 0; d < D; d++) rec.off[d] = a.off[r * D + d];
+  }
+  return rec;
+}
+
+// Returns the streamed population of direction al at DoF i from its departure record.
 template <int D, int Q, int P, typename T>
-__device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, int al, int widx, unsigned wmask) {
+__device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, int al, int widx, unsigned wmask,
+                                           const DepRec<D, T>& rec) {
   constexpr int NP = P + 1;
   constexpr int NL = (D == 3) ? NP * NP * NP : NP * NP;
   const int n0 = a.nfe[0], n1 = a.nfe[1];
-  const int64_t r = (int64_t)(al - 1) * a.n + i;
   const T* own = a.fin + (int64_t)al * a.stride;
   const bool flip = (wmask >> al) & 1u;
   const T* src = flip ? a.fin + (int64_t)opposite<Q>(al) * a.stride : own;
@@ -133,21 +162,21 @@ __device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, i
   int c;
   double lx[NP], ly[NP], lz[NP];
   if constexpr (sizeof(T) == 8) {
-    c = a.cell[r];
+    c = rec.c;
     if (c < 0) return (T)((double)own[i] + bias);  // tracker life-time limit: identity row
-    basis_xi<P>(a.gll, a.inv_den, a.xi[r * D + 0], lx);
-    basis_xi<P>(a.gll, a.inv_den, a.xi[r * D + 1], ly);
-    if constexpr (D == 3) basis_xi<P>(a.gll, a.inv_den, a.xi[r * D + 2], lz);
+    basis_xi<P>(a.gll, a.inv_den, rec.xi[0], lx);
+    basis_xi<P>(a.gll, a.inv_den, rec.xi[1], ly);
+    if constexpr (D == 3) basis_xi<P>(a.gll, a.inv_den, rec.xi[2], lz);
   } else {
-    const int code = a.code[r];
+    const int code = rec.c;
     if (code < 0) return (T)((double)own[i] + bias);
     c = code / NL;
     const int lin = code - c * NL;
     const int ax = lin % NP, ay = (lin / NP) % NP, az = (D == 3) ? lin / (NP * NP) : 0;
     constexpr double q = 1.0 / 1073741824.0;
-    basis_at<P>(a.dgl, a.inv_den, ax, (double)a.off[r * D + 0] * q, lx);
-    basis_at<P>(a.dgl, a.inv_den, ay, (double)a.off[r * D + 1] * q, ly);
-    if constexpr (D == 3) basis_at<P>(a.dgl, a.inv_den, az, (double)a.off[r * D + 2] * q, lz);
+    basis_at<P>(a.dgl, a.inv_den, ax, (double)rec.off[0] * q, lx);
+    basis_at<P>(a.dgl, a.inv_den, ay, (double)rec.off[1] * q, ly);
+    if constexpr (D == 3) basis_at<P>(a.dgl, a.inv_den, az, (double)rec.off[2] * q, lz);
   }
   const uint32_t q0 = a.divN0.div((uint32_t)c), q1 = a.divN1.div(q0);
   const int cx = c - (int)q0 * a.N[0];
@@ -248,15 +277,32 @@ __global__ __launch_bounds__((GeneralShape<Q, T>::bs)) __attribute__((amdgpu_wav
   if constexpr (GeneralShape<Q, T>::lds) {
     __shared__ T col[Q - 1][GeneralShape<Q, T>::bs];
     const int t = threadIdx.x;
+    if constexpr (sizeof(T) == 8) {
+      // FP64 records: the next direction's record is loaded before this direction's
+      // gathers, so its latency overlaps them (C3 FP64 370 -> 358 us per step; FP32, at its
+      // 80-register cap, spilled more and lost 14%)
+      DepRec<D, T> rec = load_rec<D, T>(a, i);
 #pragma unroll 1
-    for (int al = 1; al < Q; al++) col[al - 1][t] = gen_direction<D, Q, P, T>(a, i, al, widx, wmask);
+      for (int al = 1; al < Q; al++) {
+        DepRec<D, T> next = rec;
+        if (al + 1 < Q) next = load_rec<D, T>(a, (int64_t)al * a.n + i);
+        col[al - 1][t] = gen_direction<D, Q, P, T>(a, i, al, widx, wmask, rec);
+        rec = next;
+      }
+    } else {
+#pragma unroll 1
+      for (int al = 1; al < Q; al++)
+        col[al - 1][t] =
+            gen_direction<D, Q, P, T>(a, i, al, widx, wmask, load_rec<D, T>(a, (int64_t)(al - 1) * a.n + i));
+    }
     // keep the compiler from forwarding the stores into registers held across the loop
     asm volatile("" ::: "memory");
 #pragma unroll
     for (int al = 1; al < Q; al++) fs[al] = col[al - 1][t];
   } else {
 #pragma unroll
-    for (int al = 1; al < Q; al++) fs[al] = gen_direction<D, Q, P, T>(a, i, al, widx, wmask);
+    for (int al = 1; al < Q; al++)
+      fs[al] = gen_direction<D, Q, P, T>(a, i, al, widx, wmask, load_rec<D, T>(a, (int64_t)(al - 1) * a.n + i));
   }
   if (write_macro & STEP_STREAM_ONLY) {  // the SL stream with its wall values (CFDSolver::stream)
 #pragma unroll
