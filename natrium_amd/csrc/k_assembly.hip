@@ -519,7 +519,7 @@ namespace nat {
 // units: |offset| <= 1/2, resolution 9.3e-10), from which the kernel rebuilds FP64
 // weights.
 __global__ void k_records_f32(const int* __restrict__ cell, const double* __restrict__ xi, int64_t nrec, int D,
-                              int p, GllTab g, int* __restrict__ code, int* __restrict__ dxi) {
+                              int p, GllTab g, int* __restrict__ rec) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrec) return;
   const int c = cell[r];
@@ -529,11 +529,11 @@ __global__ void k_records_f32(const int* __restrict__ cell, const double* __rest
     int best = 0;
     for (int m = 1; m <= p; m++)
       if (fabs(x - g.x[m]) < fabs(x - g.x[best])) best = m;
-    dxi[r * D + d] = (int)llrint((x - g.x[best]) * 1073741824.0);
+    rec[r * (D + 1) + 1 + d] = (int)llrint((x - g.x[best]) * 1073741824.0);
     lin += best * mult;
     mult *= p + 1;
   }
-  code[r] = c < 0 ? -1 : c * mult + lin;
+  rec[r * (D + 1)] = c < 0 ? -1 : c * mult + lin;
 }
 }  // namespace nat
 
@@ -543,7 +543,7 @@ int nat_launch_records_f32(nat_ctx* ctx, int64_t nrec) {
   nat::gll_points(ctx->p, g.x, w);
   const int bs = 256;
   hipLaunchKernelGGL(nat::k_records_f32, dim3((unsigned)((nrec + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
-                     ctx->dep_cell, ctx->dep_xi64, nrec, ctx->D, ctx->p, g, ctx->dep_code32, ctx->dep_off32);
+                     ctx->dep_cell, ctx->dep_xi64, nrec, ctx->D, ctx->p, g, ctx->dep_rec32);
   NAT_HIP(hipGetLastError());
   return 0;
 }

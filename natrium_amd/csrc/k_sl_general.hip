@@ -9,7 +9,8 @@
 // every step with the same cut-off. Records: FP64 storage keeps the cell and FP64
 // unit coordinates (4 + 8D bytes); FP32 storage keeps the cell and nearest GLL node
 // packed in one int32 and the offsets from that node in 2^-30 fixed point (4 + 4D
-// bytes), instead of (p+1)^D stored weights plus column indices per row.
+// bytes, one interleaved record), instead of (p+1)^D stored weights plus column indices
+// per row.
 #include "nat_collide.hpp"
 #include "nat_fastdiv.hpp"
 
@@ -36,8 +37,8 @@ struct GeneralArgs {
   FastDivU divN0, divN1;  // division by N[0], N[1]
   const int* cell;    // FP64 storage: [(alpha-1) * n + i] departure cell (-1: identity row)
   const double* xi;   //   unit coordinates, D per record
-  const int* code;    // FP32 storage: cell * (p+1)^D + anchor node (-1: identity row)
-  const int* off;     //   fixed-point offsets from the anchor node (2^-30 units), D per record
+  const int* rec;     // FP32 storage: D + 1 int32 per record: cell * (p+1)^D + anchor node
+                      //   (-1: identity row), then the offsets from the anchor (2^-30 units)
   int dir_bytes;      // bytes of one direction's array (buffer-load range)
   const int* wall_idx;        // walls: wall entry per DoF or -1 (null: no walls)
   const unsigned* wall_flip;  //   per entry: bit alpha set when alpha reads the opposite direction
@@ -141,9 +142,17 @@ __device__ __forceinline__ DepRec<D, T> load_rec(const GeneralArgs<T>& a, int64_
 #pragma unroll
     for (int d = 0; d < D; d++) rec.xi[d] = a.xi[r * D + d];
   } else {
-    rec.c = a.code[r];
-#pragma unroll
-    for (int d = 0; d < D; d++) rec.off[d] = a.off[r * D + d];
+    if constexpr (D == 3) {  // one 16-byte load per record
+      const int4 v = reinterpret_cast<const int4*>(a.rec)[r];
+      rec.c = v.x;
+      rec.off[0] = v.y;
+      rec.off[1] = v.z;
+      rec.off[2] = v.w;
+    } else {
+      rec.c = a.rec[r * 3];
+      rec.off[0] = a.rec[r * 3 + 1];
+      rec.off[1] = a.rec[r * 3 + 2];
+    }
   }
   return rec;
 }
@@ -362,8 +371,7 @@ static void fill_general(nat_ctx* ctx, GeneralArgs<T>& a) {
   a.divN1.set((uint32_t)a.N[1]);
   a.cell = ctx->dep_cell;
   a.xi = ctx->dep_xi64;
-  a.code = ctx->dep_code32;
-  a.off = ctx->dep_off32;
+  a.rec = ctx->dep_rec32;
   a.dir_bytes = (int)(ctx->stride * (int64_t)sizeof(T));
   a.wall_idx = ctx->has_walls ? ctx->wall_idx : nullptr;
   a.wall_flip = ctx->wall_flip;

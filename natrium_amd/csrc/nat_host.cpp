@@ -517,8 +517,7 @@ int assemble_general(nat_ctx* ctx) {
   }
   TRY(nat_launch_assemble_general(ctx, ddisp, ddirw, ctx->dep_cell, ctx->dep_xi64, dmask, dbias));
   if (ctx->fp32) {
-    if (!ctx->dep_off32) TRY(alloc(ctx, &ctx->dep_off32, nrec * ctx->D));
-    if (!ctx->dep_code32) TRY(alloc(ctx, &ctx->dep_code32, nrec));
+    if (!ctx->dep_rec32) TRY(alloc(ctx, &ctx->dep_rec32, nrec * (ctx->D + 1)));
     const int64_t nl =
         ctx->D == 3 ? (int64_t)(ctx->p + 1) * (ctx->p + 1) * (ctx->p + 1) : (int64_t)(ctx->p + 1) * (ctx->p + 1);
     if ((int64_t)ctx->desc.ncells[0] * ctx->desc.ncells[1] * ctx->desc.ncells[2] * nl >= (int64_t)1 << 31)
@@ -588,7 +587,7 @@ void nat_destroy(nat_ctx* ctx) {
   nat_drop_graphs(ctx);
   void* bufs[] = {ctx->f[0],     ctx->f[1],        ctx->g[0],     ctx->g[1],       ctx->rho,       ctx->u,
                   ctx->T,        ctx->mask,        ctx->d_vert,   ctx->d_gll,      ctx->d_coords,  ctx->d_owner,
-                  ctx->d_status, ctx->d_sums,      ctx->dep_cell, ctx->dep_xi64,   ctx->dep_off32, ctx->dep_code32, ctx->ax_w,
+                  ctx->d_status, ctx->d_sums,      ctx->dep_cell, ctx->dep_xi64,   ctx->dep_rec32, ctx->ax_w,
                   ctx->ax_off,   ctx->sedg_df,     ctx->sedg_geo, ctx->comp_tab,    ctx->wall_idx,
                   ctx->wall_flip, ctx->wall_bias};
   for (void* b : bufs)

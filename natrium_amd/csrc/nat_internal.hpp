@@ -162,8 +162,9 @@ struct nat_ctx {
   // SL general backend: departure records per (alpha-1, node)
   int* dep_cell = nullptr;
   double* dep_xi64 = nullptr;  // D per record
-  int* dep_off32 = nullptr;    // FP32 storage: fixed-point offset from the anchor node (2^-30 units), D per record
-  int* dep_code32 = nullptr;   // FP32: cell * (p+1)^D + anchor
+  // FP32 storage: per record D + 1 int32, interleaved so that one load brings a record: the
+  // code cell * (p+1)^D + anchor node, then the fixed-point offsets from the anchor (2^-30 units)
+  int* dep_rec32 = nullptr;
 
   // velocity-neq bounce-back walls (general SL backend): per node the index of its wall
   // entry or -1; per wall entry a mask of the directions read from the opposite
