@@ -362,9 +362,10 @@ class CFDSolver {
   }
   // output (CFDSolver.cpp:947-1100): results table, solution files and checkpoints
   void output(size_t it, bool is_final) {
-    if (cfg_.outputTableInterval && (it % cfg_.outputTableInterval == 0 || is_final) && !cfg_.tableFile.empty())
-      c_->check(nat_write_stats(c_->get(), cfg_.tableFile.c_str(), (int64_t)it, tableStarted_ ? 0 : 1), Seam::Solver),
-          tableStarted_ = true;
+    if (cfg_.outputTableInterval && (it % cfg_.outputTableInterval == 0 || is_final) && !cfg_.tableFile.empty()) {
+      c_->check(nat_write_stats(c_->get(), cfg_.tableFile.c_str(), (int64_t)it, tableStarted_ ? 0 : 1), Seam::Solver);
+      tableStarted_ = true;
+    }
     if (cfg_.outputSolutionInterval && (it % cfg_.outputSolutionInterval == 0 || is_final) &&
         !cfg_.outputDirectory.empty())
       c_->check(nat_write_vtu(c_->get(), cfg_.outputDirectory.c_str(), (int64_t)it), Seam::Solver);
