@@ -225,14 +225,25 @@ int nat_struct_tile(nat_ctx* ctx, int* tile /* 3 */);
 int nat_pack_key(nat_ctx* ctx, int64_t* key);
 int nat_step_packed(nat_ctx* const* ctxs, int n, int nsteps);
 
-/* Tile autotuner, no counterpart in the reference: times every tile shape the structured SL
+/* Launch-shape autotuner, no counterpart in the reference: times every launch shape the SL
  * step kernel is built with for this (stencil, order) on the context's own mesh, nsteps steps
  * each (nsteps <= 0: 10), and keeps the fastest for this context and, process-wide, for later
  * contexts of the same kernel instantiation and mesh size (which then skip the timing). The
  * timing launches write only the idle population buffer: populations, fields and time are
- * unchanged, and results do not depend on the tile. Structured SL contexts only; tile receives
- * the chosen (x, y, z) extents. */
+ * unchanged, and results do not depend on the shape. SL contexts (not compressible):
+ *   structured backend: the tiles; tile receives the chosen (x, y, z) extents;
+ *   general backend: with FP32 storage in 3D the six-wave form and the DEEP form (4 waves,
+ *     more source rows and the next departure record in flight; by default chosen when its
+ *     grid fits one round of the device); tile receives the chosen form's (DoFs per block,
+ *     waves per SIMD it is built for, z-planes of source rows loaded together). */
 int nat_autotune(nat_ctx* ctx, int nsteps, int* tile /* 3, may be NULL */);
+
+/* Pins the general SL backend's kernel form, no counterpart in the reference: form -1 the
+ * default rule (DEEP when its grid fits one round of the device), 0 the six-wave form, 1 the
+ * DEEP form (FP32 storage in 3D; other contexts have one form and accept only -1 and 0). Both
+ * forms give bit-identical results; the choice only moves the step time. NAT_ERR_ARG for
+ * another value or backend. */
+int nat_set_general_form(nat_ctx* ctx, int form);
 
 /* ---- Output and restart (host side, from fields downloaded through this API) ---- */
 

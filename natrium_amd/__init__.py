@@ -58,7 +58,7 @@ EXPORTS = [
     "nat_step", "nat_sl_assemble", "nat_sl_bgk_step", "nat_sedg_rk5_step", "nat_compressible_step",
     "nat_step_timed", "nat_get_macro", "nat_get_knudsen", "nat_sums", "nat_sums_noalloc", "nat_sync",
     "nat_stats", "nat_write_stats", "nat_write_vtu", "nat_checkpoint_write", "nat_checkpoint_read",
-    "nat_struct_tile", "nat_pack_key", "nat_step_packed", "nat_autotune",
+    "nat_struct_tile", "nat_pack_key", "nat_step_packed", "nat_autotune", "nat_set_general_form",
 ]
 
 _lib = None
@@ -138,6 +138,8 @@ def lib():
     L.nat_step_packed.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int]
     L.nat_autotune.restype = ctypes.c_int
     L.nat_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.nat_set_general_form.restype = ctypes.c_int
+    L.nat_set_general_form.argtypes = [vp, ctypes.c_int]
     _lib = L
     return L
 
@@ -383,11 +385,17 @@ class Solver:
         return k.value
 
     def autotune(self, nsteps=10):
-        """Time the structured kernel's tile shapes on this mesh and keep the fastest
-        (nat_autotune); returns the chosen tile."""
+        """Time the SL kernel's launch shapes on this mesh and keep the fastest (nat_autotune);
+        returns the chosen tile (structured backend) or (DoFs per block, waves per SIMD,
+        z-planes in flight) of the chosen form (general backend)."""
         t = (ctypes.c_int * 3)()
         self._check(lib().nat_autotune(self._h, int(nsteps), t))
         return tuple(t)
+
+    def set_general_form(self, form):
+        """Pin the general backend's kernel form (nat_set_general_form): -1 default rule,
+        0 six-wave form, 1 DEEP form (FP32 storage, 3D)."""
+        self._check(lib().nat_set_general_form(self._h, int(form)))
 
 
 def step_packed(solvers, nsteps):

@@ -14,6 +14,11 @@
 #include "nat_collide.hpp"
 #include "nat_fastdiv.hpp"
 
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <type_traits>
+
 
 namespace nat {
 
@@ -46,6 +51,41 @@ struct GeneralArgs {
   double gll[MAXP + 1];
   double inv_den[MAXP + 1];
   double dgl[MAXP + 1][MAXP + 1];  // x_a - x_k
+};
+
+// Launch shape. The direction loop is bound by the latency of its gathers, so the
+// kernel wants many waves per SIMD; holding the Q streamed populations in registers
+// across that loop forces either few waves or scratch spills. Where Q populations of
+// a block fit in LDS at >= 4 waves per SIMD (FP32 storage; FP64 up to Q = 19), each
+// thread parks its populations in its own LDS column (no barrier: no thread reads
+// another's column), the direction loop stays rolled, and the collision reloads the
+// column. FP32: 6 waves (80 registers); FP64: the 4 waves its LDS allows. FP64 D3Q27
+// (216 B per DoF) keeps the populations in registers at 6 waves. On 24^3 perturbed
+// cells, p = 3, this shape gives D3Q19 FP32 1.25 -> 1.58 and D3Q27 FP32 0.86 -> 1.25
+// GLUPS over the unrolled register form.
+//
+// Within one direction the rows of `zu` z-planes of the source block are loaded
+// together: one plane at 80 registers, as many planes as keep the raw rows within 64
+// registers at 4 waves (C3 FP64, two planes: 358 -> 342 us per step). The DEEP form of
+// the FP32 3D kernel runs at 4 waves with those planes in flight and the next
+// direction's record prefetched; it is launched when the grid fits one round of it,
+// where each wave's dependent loads, not the number of waves, set the time (D3Q27 p = 3
+// on 16^3 perturbed cells, 460 blocks: 156 -> 138 us per step). A grid that fills the
+// 6-wave form (C3: 1520 blocks) keeps that form (the 4-wave one: 205 -> 245 us).
+template <int D, int Q, int P, typename T, bool DEEP>
+struct GeneralShape {
+  static constexpr bool lds = sizeof(T) == 4 || Q * 8 <= 160;
+  static constexpr int waves = DEEP ? 4 : (sizeof(T) == 4 ? 6 : (lds ? 4 : 6));
+  // DoFs per block: 128 with FP64 storage (C3 FP64 +6% over 256), 256 with FP32
+  // (128 measured 18% slower on D3Q27)
+  static constexpr int bs = sizeof(T) == 8 ? 128 : 256;
+  static constexpr int nz = D == 3 ? P + 1 : 1;
+  static constexpr int plane_regs = (P + 1) * (P + 1) * (int)(sizeof(T) / 4);
+  static constexpr int zfit = 64 / plane_regs < 1 ? 1 : (64 / plane_regs > nz ? nz : 64 / plane_regs);
+  static constexpr int zu = (waves == 4 && lds) ? zfit : 1;
+  // the next direction's record loaded before this direction's gathers (FP32 at 6 waves:
+  // spills and loses 14%)
+  static constexpr bool prefetch = sizeof(T) == 8 || DEEP;
 };
 
 // Lagrange values at x_a + d (anchor a), in FP64: prod_{k != j} (x_a - x_k + d) / den_j
@@ -158,7 +198,7 @@ __device__ __forceinline__ DepRec<D, T> load_rec(const GeneralArgs<T>& a, int64_
 }
 
 // Returns the streamed population of direction al at DoF i from its departure record.
-template <int D, int Q, int P, typename T>
+template <int D, int Q, int P, typename T, int ZU>
 __device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, int al, int widx, unsigned wmask,
                                            const DepRec<D, T>& rec) {
   constexpr int NP = P + 1;
@@ -215,19 +255,25 @@ __device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, i
         (void*)(((uint64_t)phi << 32) | (uint64_t)plo), (short)0, __builtin_amdgcn_readfirstlane(a.dir_bytes),
         0x00020000);
     const int b0 = (int)base * (int)sizeof(T);
+    constexpr int NZ = (D == 3) ? NP : 1;
 #pragma unroll 1
-    for (int mz = 0; mz < ((D == 3) ? NP : 1); mz++) {
-      double accz = 0;
+    for (int mz0 = 0; mz0 < NZ; mz0 += ZU) {
 #pragma unroll
-      for (int my = 0; my <= P; my++) {
-        double v[NP];
-        load_row<P, T>(rs, b0 + (int)sizeof(T) * (n0 * (my + n1 * mz)), v);
-        double row = lx[0] * v[0];
+      for (int dz = 0; dz < ZU; dz++) {
+        const int mz = mz0 + dz;
+        if (NZ % ZU != 0 && mz >= NZ) break;
+        double accz = 0;
 #pragma unroll
-        for (int mx = 1; mx <= P; mx++) row += lx[mx] * v[mx];
-        accz += ly[my] * row;
+        for (int my = 0; my <= P; my++) {
+          double v[NP];
+          load_row<P, T>(rs, b0 + (int)sizeof(T) * (n0 * (my + n1 * mz)), v);
+          double row = lx[0] * v[0];
+#pragma unroll
+          for (int mx = 1; mx <= P; mx++) row += lx[mx] * v[mx];
+          accz += ly[my] * row;
+        }
+        acc += ((D == 3) ? lz[mz] : 1.0) * accz;
       }
-      acc += ((D == 3) ? lz[mz] : 1.0) * accz;
     }
   } else if constexpr (D == 2) {
 #pragma unroll
@@ -251,27 +297,8 @@ __device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, i
   return (T)(acc + bias);
 }
 
-// Launch shape. The direction loop is bound by the latency of its gathers, so the
-// kernel wants many waves per SIMD; holding the Q streamed populations in registers
-// across that loop forces either few waves or scratch spills. Where Q populations of
-// a block fit in LDS at >= 4 waves per SIMD (FP32 storage; FP64 up to Q = 19), each
-// thread parks its populations in its own LDS column (no barrier: no thread reads
-// another's column), the direction loop and the z-planes of the interpolation stay
-// rolled, and the collision reloads the column. FP32: 6 waves (80 registers); FP64:
-// the 4 waves its LDS allows. FP64 D3Q27 (216 B per DoF) keeps the populations in
-// registers at 6 waves. On 24^3 perturbed cells, p = 3, this shape gives D3Q19 FP32
-// 1.25 -> 1.58 and D3Q27 FP32 0.86 -> 1.25 GLUPS over the unrolled register form.
-template <int Q, typename T>
-struct GeneralShape {
-  static constexpr bool lds = sizeof(T) == 4 || Q * 8 <= 160;
-  static constexpr int waves = sizeof(T) == 4 ? 6 : (lds ? 4 : 6);
-  // DoFs per block: 128 with FP64 storage (C3 FP64 +6% over 256), 256 with FP32
-  // (128 measured 18% slower on D3Q27)
-  static constexpr int bs = sizeof(T) == 8 ? 128 : 256;
-};
-
-template <int D, int Q, int P, typename T>
-__global__ __launch_bounds__((GeneralShape<Q, T>::bs)) __attribute__((amdgpu_waves_per_eu(GeneralShape<Q, T>::waves))) void k_sl_general(
+template <int D, int Q, int P, typename T, bool DEEP>
+__global__ __launch_bounds__((GeneralShape<D, Q, P, T, DEEP>::bs)) __attribute__((amdgpu_waves_per_eu(GeneralShape<D, Q, P, T, DEEP>::waves))) void k_sl_general(
     GeneralArgs<T> a, CollideArgs<T> ca, int write_macro) {
   const int64_t i = xcd_block_index() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
@@ -283,10 +310,11 @@ __global__ __launch_bounds__((GeneralShape<Q, T>::bs)) __attribute__((amdgpu_wav
   }
   T fs[Q];
   fs[0] = a.fin[i];
-  if constexpr (GeneralShape<Q, T>::lds) {
-    __shared__ T col[Q - 1][GeneralShape<Q, T>::bs];
+  using S = GeneralShape<D, Q, P, T, DEEP>;
+  if constexpr (S::lds) {
+    __shared__ T col[Q - 1][S::bs];
     const int t = threadIdx.x;
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (S::prefetch) {
       // FP64 records: the next direction's record is loaded before this direction's
       // gathers, so its latency overlaps them (C3 FP64 370 -> 358 us per step; FP32, at its
       // 80-register cap, spilled more and lost 14%)
@@ -295,14 +323,14 @@ __global__ __launch_bounds__((GeneralShape<Q, T>::bs)) __attribute__((amdgpu_wav
       for (int al = 1; al < Q; al++) {
         DepRec<D, T> next = rec;
         if (al + 1 < Q) next = load_rec<D, T>(a, (int64_t)al * a.n + i);
-        col[al - 1][t] = gen_direction<D, Q, P, T>(a, i, al, widx, wmask, rec);
+        col[al - 1][t] = gen_direction<D, Q, P, T, S::zu>(a, i, al, widx, wmask, rec);
         rec = next;
       }
     } else {
 #pragma unroll 1
       for (int al = 1; al < Q; al++)
         col[al - 1][t] =
-            gen_direction<D, Q, P, T>(a, i, al, widx, wmask, load_rec<D, T>(a, (int64_t)(al - 1) * a.n + i));
+            gen_direction<D, Q, P, T, S::zu>(a, i, al, widx, wmask, load_rec<D, T>(a, (int64_t)(al - 1) * a.n + i));
     }
     // keep the compiler from forwarding the stores into registers held across the loop
     asm volatile("" ::: "memory");
@@ -311,7 +339,7 @@ __global__ __launch_bounds__((GeneralShape<Q, T>::bs)) __attribute__((amdgpu_wav
   } else {
 #pragma unroll
     for (int al = 1; al < Q; al++)
-      fs[al] = gen_direction<D, Q, P, T>(a, i, al, widx, wmask, load_rec<D, T>(a, (int64_t)(al - 1) * a.n + i));
+      fs[al] = gen_direction<D, Q, P, T, S::zu>(a, i, al, widx, wmask, load_rec<D, T>(a, (int64_t)(al - 1) * a.n + i));
   }
   if (write_macro & STEP_STREAM_ONLY) {  // the SL stream with its wall values (CFDSolver::stream)
 #pragma unroll
@@ -321,20 +349,41 @@ __global__ __launch_bounds__((GeneralShape<Q, T>::bs)) __attribute__((amdgpu_wav
   bgk_collide_store_pairs<Q, T>(fs, ca, i, (write_macro & STEP_MACRO) != 0, a.fout, a.stride);
 }
 
-template <int D, int Q, int P, typename T>
-static int launch_general(nat_ctx* ctx, const GeneralArgs<T>& a, const CollideArgs<T>& ca, int wm) {
-  constexpr int bs = GeneralShape<Q, T>::bs;
-  hipLaunchKernelGGL((k_sl_general<D, Q, P, T>), dim3((unsigned)((a.n + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
-                     a, ca, wm);
+template <int D, int Q, int P, typename T, bool DEEP>
+static int launch_general_shape(nat_ctx* ctx, const GeneralArgs<T>& a, const CollideArgs<T>& ca, int wm) {
+  constexpr int bs = GeneralShape<D, Q, P, T, DEEP>::bs;
+  hipLaunchKernelGGL((k_sl_general<D, Q, P, T, DEEP>), dim3((unsigned)((a.n + bs - 1) / bs)), dim3(bs), 0,
+                     ctx->stream, a, ca, wm);
   NAT_HIP(hipGetLastError());
   return 0;
 }
 
-template <typename T>
-static int dispatch_general(nat_ctx* ctx, const GeneralArgs<T>& a, const CollideArgs<T>& ca, int wm) {
+// The DEEP form (FP32 3D only) when the autotuner chose it or, by default, when its grid fits
+// one round: 4 blocks of 256 DoFs per CU at 4 waves per SIMD.
+template <int D, int Q, int P, typename T>
+static bool deep_form(const nat_ctx* ctx, int64_t n) {
+  if constexpr (sizeof(T) == 4 && D == 3) {
+    constexpr int bs = GeneralShape<D, Q, P, T, true>::bs;
+    const int64_t blocks = (n + bs - 1) / bs;
+    return ctx->gen_pref >= 0 ? ctx->gen_pref == 1 : (ctx->n_cu > 0 && blocks <= 4 * (int64_t)ctx->n_cu);
+  } else {
+    return false;
+  }
+}
+
+template <int D, int Q, int P, typename T>
+static int launch_general(nat_ctx* ctx, const GeneralArgs<T>& a, const CollideArgs<T>& ca, int wm) {
+  if (deep_form<D, Q, P, T>(ctx, a.n)) return launch_general_shape<D, Q, P, T, true>(ctx, a, ca, wm);
+  return launch_general_shape<D, Q, P, T, false>(ctx, a, ca, wm);
+}
+
+// Calls f(D, Q, P) (as integral constants) for the context's instantiation.
+template <typename F>
+static int dispatch_dqp(nat_ctx* ctx, F&& f) {
   const int Q = ctx->Q, P = ctx->p;
 #define NAT_G(DD, QQ, PP) \
-  if (Q == QQ && P == PP) return launch_general<DD, QQ, PP, T>(ctx, a, ca, wm);
+  if (Q == QQ && P == PP)   \
+    return f(std::integral_constant<int, DD>{}, std::integral_constant<int, QQ>{}, std::integral_constant<int, PP>{});
   NAT_G(2, 9, 1)
   NAT_G(2, 9, 2)
   NAT_G(2, 9, 3)
@@ -354,6 +403,13 @@ static int dispatch_general(nat_ctx* ctx, const GeneralArgs<T>& a, const Collide
 #undef NAT_G
   ctx->err = "general SL kernel not instantiated for this (stencil, order)";
   return NAT_ERR_UNSUPPORTED;
+}
+
+template <typename T>
+static int dispatch_general(nat_ctx* ctx, const GeneralArgs<T>& a, const CollideArgs<T>& ca, int wm) {
+  return dispatch_dqp(ctx, [&](auto d, auto q, auto p) {
+    return launch_general<decltype(d)::value, decltype(q)::value, decltype(p)::value, T>(ctx, a, ca, wm);
+  });
 }
 
 template <typename T>
@@ -391,14 +447,15 @@ static void fill_general(nat_ctx* ctx, GeneralArgs<T>& a) {
 
 }  // namespace nat
 
-int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
+namespace {
+
+// One fused step from f[cur] into f[1 - cur], without advancing cur.
+int general_launch(nat_ctx* ctx, int write_macro) {
   using namespace nat;
   if (ctx->stride * (int64_t)(ctx->fp32 ? 4 : 8) > INT32_MAX) {
     ctx->err = "general backend: one direction's array exceeds 2 GiB";
     return NAT_ERR_UNSUPPORTED;
   }
-  const int dst = 1 - ctx->cur;
-  int rc;
   if (ctx->fp32) {
     GeneralArgs<float> a;
     fill_general<float>(ctx, a);
@@ -412,22 +469,124 @@ int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
     ca.mstride = ctx->n_store;
     ca.status = ctx->d_status;
     ca.stab = ctx->stab_on();
-    rc = dispatch_general<float>(ctx, a, ca, write_macro);
-  } else {
-    GeneralArgs<double> a;
-    fill_general<double>(ctx, a);
-    CollideArgs<double> ca;
-    ca.omega = 1.0 / ctx->tau;
-    ca.omega64 = 1.0 / ctx->tau;
-    ca.rho0 = 0.0;
-    ca.scaling = ctx->st.s;
-    ca.rho = (double*)ctx->rho;
-    ca.u = (double*)ctx->u;
-    ca.mstride = ctx->n_store;
-    ca.status = ctx->d_status;
-    ca.stab = ctx->stab_on();
-    rc = dispatch_general<double>(ctx, a, ca, write_macro);
+    return dispatch_general<float>(ctx, a, ca, write_macro);
   }
-  if (rc == 0) ctx->cur = dst;
+  GeneralArgs<double> a;
+  fill_general<double>(ctx, a);
+  CollideArgs<double> ca;
+  ca.omega = 1.0 / ctx->tau;
+  ca.omega64 = 1.0 / ctx->tau;
+  ca.rho0 = 0.0;
+  ca.scaling = ctx->st.s;
+  ca.rho = (double*)ctx->rho;
+  ca.u = (double*)ctx->u;
+  ca.mstride = ctx->n_store;
+  ca.status = ctx->d_status;
+  ca.stab = ctx->stab_on();
+  return dispatch_general<double>(ctx, a, ca, write_macro);
+}
+
+struct GenKey {
+  int Q, p, walls;
+  int64_t n;
+  bool operator<(const GenKey& o) const {
+    return std::tie(Q, p, walls, n) < std::tie(o.Q, o.p, o.walls, o.n);
+  }
+};
+std::mutex g_gen_mutex;
+std::map<GenKey, int> g_gen_cache;
+
+}  // namespace
+
+int nat_launch_sl_general(nat_ctx* ctx, int write_macro) {
+  const int rc = general_launch(ctx, write_macro);
+  if (rc == 0) ctx->cur = 1 - ctx->cur;
   return rc;
+}
+
+// Form of the general SL kernel picked by measurement. FP32 3D contexts have two (the
+// six-wave form and the DEEP form): each is timed for nsteps steps on the context's own mesh,
+// writing the idle ping-pong buffer (populations, fields and time stay as they were), and the
+// faster is kept for this context and, process-wide, for later contexts of the same
+// instantiation and DoF count. Both forms run the same arithmetic in the same order, so
+// results do not depend on the choice. tile receives the chosen form's DoFs per block, waves
+// per SIMD it is built for, and z-planes of source rows loaded together.
+int nat_internal_autotune_general(nat_ctx* ctx, int nsteps, int* tile, double* best_us) {
+  using namespace nat;
+  if (!ctx->assembled) return NAT_ERR_STATE;
+  if (best_us) *best_us = 0;
+  if (ctx->fp32 && ctx->D == 3) {
+    const GenKey key{ctx->Q, ctx->p, ctx->has_walls ? 1 : 0, ctx->n_dofs};
+    int cached = -1;
+    {
+      std::lock_guard<std::mutex> lk(g_gen_mutex);
+      auto it = g_gen_cache.find(key);
+      if (it != g_gen_cache.end()) cached = it->second;
+    }
+    if (cached >= 0) {
+      ctx->gen_pref = cached;
+    } else {
+      if (nsteps < 1) nsteps = 10;
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      NAT_HIP(hipEventCreate(&e0));
+      if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        ctx->err = "hipEventCreate failed";
+        return NAT_ERR_HIP;
+      }
+      int best = -1, rc = 0;
+      float best_ms = 0;
+      for (int pref = 0; pref < 2 && !rc; pref++) {
+        ctx->gen_pref = pref;
+        for (int s = 0; s < 2 && !rc; s++) rc = general_launch(ctx, 0);
+        if (rc) break;
+        if (hipEventRecord(e0, ctx->stream) != hipSuccess) {
+          ctx->err = "autotune timing events failed";
+          rc = NAT_ERR_HIP;
+          break;
+        }
+        for (int s = 0; s < nsteps && !rc; s++) rc = general_launch(ctx, 0);
+        if (rc) break;
+        float ms = 0;
+        if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          ctx->err = "autotune timing events failed";
+          rc = NAT_ERR_HIP;
+          break;
+        }
+        if (best < 0 || ms < best_ms) {
+          best = pref;
+          best_ms = ms;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      if (rc) {
+        ctx->gen_pref = -1;
+        return rc;
+      }
+      ctx->gen_pref = best;
+      if (best_us) *best_us = 1e3 * best_ms / nsteps;
+      std::lock_guard<std::mutex> lk(g_gen_mutex);
+      g_gen_cache[key] = best;
+    }
+  }
+  return dispatch_dqp(ctx, [&](auto d, auto q, auto p) {
+    constexpr int D = decltype(d)::value, Q = decltype(q)::value, P = decltype(p)::value;
+    auto put = [&](auto shape) {
+      using S = decltype(shape);
+      tile[0] = S::bs;
+      tile[1] = S::waves;
+      tile[2] = S::zu;
+    };
+    if (ctx->fp32) {
+      if (deep_form<D, Q, P, float>(ctx, ctx->n_dofs))
+        put(GeneralShape<D, Q, P, float, true>{});
+      else
+        put(GeneralShape<D, Q, P, float, false>{});
+    } else {
+      put(GeneralShape<D, Q, P, double, false>{});
+    }
+    return 0;
+  });
 }

@@ -582,7 +582,7 @@ int time_tile(nat_ctx* ctx, int nsteps, hipEvent_t e0, hipEvent_t e1, float* ms)
 // tile: each node's interpolation and collision run the same arithmetic in any tile.
 int nat_internal_autotune(nat_ctx* ctx, int nsteps, int* tile, double* best_us) {
   if (ctx->desc.scheme != NAT_SL || ctx->desc.compressible || ctx->backend != NAT_BACKEND_STRUCTURED) {
-    ctx->err = "nat_autotune: the tile autotuner serves the structured SL backend";
+    ctx->err = "nat_autotune: the autotuner serves the structured and general SL backends";
     return NAT_ERR_ARG;
   }
   if (!ctx->assembled) return NAT_ERR_STATE;

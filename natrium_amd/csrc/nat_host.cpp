@@ -24,6 +24,7 @@ int nat_launch_sl_struct_tile(nat_ctx* ctx, int* tile);
 int nat_internal_pack_key(nat_ctx* ctx, int64_t* key);
 int nat_internal_pack_step(nat_ctx* const* ctxs, int n, int nsteps, hipStream_t stream);
 int nat_internal_autotune(nat_ctx* ctx, int nsteps, int* tile, double* best_us);
+int nat_internal_autotune_general(nat_ctx* ctx, int nsteps, int* tile, double* best_us);
 int nat_launch_sl_general(nat_ctx* ctx, int write_macro);
 int nat_launch_collide(nat_ctx* ctx, int init);
 int nat_launch_stabilize(nat_ctx* ctx);
@@ -673,6 +674,7 @@ int nat_create(const nat_desc* desc, nat_ctx** out) {
   NAT_HIP(hipEventCreate(&ctx->ev[0]));
   NAT_HIP(hipEventCreate(&ctx->ev[1]));
   TRY(nat_sums_scratch_setup(ctx));  // this device's address of the fixed sums scratch
+  NAT_HIP(hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device));
   TRY(alloc(ctx, &ctx->d_vert, ctx->verts.size()));
   NAT_HIP(hipMemcpy(ctx->d_vert, ctx->verts.data(), ctx->verts.size() * sizeof(double), hipMemcpyHostToDevice));
   double gll[MAXP + 1], gw[MAXP + 1];
@@ -1285,10 +1287,22 @@ int nat_step_packed(nat_ctx* const* ctxs, int n, int nsteps) {
   return 0;
 }
 
+int nat_set_general_form(nat_ctx* ctx, int form) {
+  if (!(ctx->desc.scheme == NAT_SL && !ctx->desc.compressible && ctx->backend == NAT_BACKEND_GENERAL))
+    return fail(ctx, NAT_ERR_ARG, "nat_set_general_form: a general-backend SL context is needed");
+  const bool two = ctx->fp32 && ctx->D == 3;
+  if (form < -1 || form > (two ? 1 : 0)) return fail(ctx, NAT_ERR_ARG, "nat_set_general_form: form out of range");
+  if (form != ctx->gen_pref) nat_drop_graphs(ctx);  // captured steps hold the other form's launches
+  ctx->gen_pref = form;
+  return 0;
+}
+
 int nat_autotune(nat_ctx* ctx, int nsteps, int* tile) {
   nat_drop_graphs(ctx);  // captured steps hold the previous tile's launches
   int t[3] = {0, 0, 0};
-  const int rc = nat_internal_autotune(ctx, nsteps, t, nullptr);
+  const bool general = ctx->desc.scheme == NAT_SL && !ctx->desc.compressible && ctx->backend == NAT_BACKEND_GENERAL;
+  const int rc = general ? nat_internal_autotune_general(ctx, nsteps, t, nullptr)
+                         : nat_internal_autotune(ctx, nsteps, t, nullptr);
   if (rc) return rc;
   if (tile)
     for (int i = 0; i < 3; i++) tile[i] = t[i];

@@ -139,6 +139,7 @@ struct nat_ctx {
   // meanwhile (a second context being set up)
   hipStream_t cap_stream = nullptr;
   int device = -1;  // HIP device current at nat_create (the context's allocations live there)
+  int n_cu = 0;     // compute units of that device
 
   // device state
   void* f[2] = {nullptr, nullptr};
@@ -196,6 +197,7 @@ struct nat_ctx {
   int* ax_off = nullptr;  // [3][MAXSHIFT][MAXP] source cell offsets
   bool ax_sym = false;    // tables equal on all axes and mirror symmetric in the shift sign
   int tile_pref = -1;     // structured tile: -1 size heuristic, 0 large tile, 1 small tile (autotuner)
+  int gen_pref = -1;      // general SL FP32 3D kernel: -1 grid-fill rule, 0 six-wave form, 1 DEEP form (autotuner)
   int sid[nat::MAXQ][3] = {};
   int off_lo[3][nat::MAXSHIFT] = {}, off_hi[3][nat::MAXSHIFT] = {};
   int nshift[3] = {0, 0, 0};
