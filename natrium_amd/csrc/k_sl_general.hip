@@ -197,28 +197,21 @@ __device__ __forceinline__ DepRec<D, T> load_rec(const GeneralArgs<T>& a, int64_
   return rec;
 }
 
-// Returns the streamed population of direction al at DoF i from its departure record.
-template <int D, int Q, int P, typename T, int ZU>
-__device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, int al, int widx, unsigned wmask,
-                                           const DepRec<D, T>& rec) {
+// 1-D Lagrange weights of a departure record in FP64 and the lattice index of its cell's
+// first node (records with rec.c >= 0 only).
+template <int D, int P, typename T>
+__device__ __forceinline__ int64_t dep_weights(const GeneralArgs<T>& a, const DepRec<D, T>& rec, double (&lx)[P + 1],
+                                               double (&ly)[P + 1], double (&lz)[P + 1]) {
   constexpr int NP = P + 1;
   constexpr int NL = (D == 3) ? NP * NP * NP : NP * NP;
-  const int n0 = a.nfe[0], n1 = a.nfe[1];
-  const T* own = a.fin + (int64_t)al * a.stride;
-  const bool flip = (wmask >> al) & 1u;
-  const T* src = flip ? a.fin + (int64_t)opposite<Q>(al) * a.stride : own;
-  const double bias = widx >= 0 ? a.wall_bias[(int64_t)widx * Q + al] : 0.0;
   int c;
-  double lx[NP], ly[NP], lz[NP];
   if constexpr (sizeof(T) == 8) {
     c = rec.c;
-    if (c < 0) return (T)((double)own[i] + bias);  // tracker life-time limit: identity row
     basis_xi<P>(a.gll, a.inv_den, rec.xi[0], lx);
     basis_xi<P>(a.gll, a.inv_den, rec.xi[1], ly);
     if constexpr (D == 3) basis_xi<P>(a.gll, a.inv_den, rec.xi[2], lz);
   } else {
     const int code = rec.c;
-    if (code < 0) return (T)((double)own[i] + bias);
     c = code / NL;
     const int lin = code - c * NL;
     const int ax = lin % NP, ay = (lin / NP) % NP, az = (D == 3) ? lin / (NP * NP) : 0;
@@ -231,11 +224,68 @@ __device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, i
   const int cx = c - (int)q0 * a.N[0];
   const int cy = (int)q0 - (int)q1 * a.N[1];
   const int cz = (D == 3) ? (int)q1 : 0;
-  const int64_t base = (int64_t)cx * P + (int64_t)n0 * ((int64_t)cy * P + (int64_t)n1 * ((int64_t)cz * P));
+  return (int64_t)cx * P + (int64_t)a.nfe[0] * ((int64_t)cy * P + (int64_t)a.nfe[1] * ((int64_t)cz * P));
+}
+
+// Buffer resource over one direction's population array (wave-uniform base).
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t dir_rsrc(const GeneralArgs<T>& a, const T* arr) {
+  const uint64_t pb = (uint64_t)arr;
+  const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pb >> 32));
+  const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pb);
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)phi << 32) | (uint64_t)plo), (short)0,
+                                           __builtin_amdgcn_readfirstlane(a.dir_bytes), 0x00020000);
+}
+
+// The row of the assembled operator under the reference cut-off: every product weight
+// (same factors, same order) is formed and the terms with |w| < 1e-10 dropped, the source
+// rows read through buffer loads.
+template <int D, int P, typename T>
+__device__ __forceinline__ double kept_sum(__amdgpu_buffer_rsrc_t rs, int b0, int n0, int n1,
+                                           const double (&lx)[P + 1], const double (&ly)[P + 1],
+                                           const double (&lz)[P + 1]) {
+  constexpr int NP = P + 1;
+  constexpr int NZ = (D == 3) ? NP : 1;
+  double kept = 0;
+#pragma unroll 1
+  for (int mz = 0; mz < NZ; mz++) {
+#pragma unroll 1
+    for (int my = 0; my <= P; my++) {
+      double v[NP];
+      load_row<P, T>(rs, b0 + (int)sizeof(T) * (n0 * (my + n1 * mz)), v);
+#pragma unroll
+      for (int mx = 0; mx <= P; mx++) {
+        const double w = (D == 3) ? lx[mx] * ly[my] * lz[mz] : lx[mx] * ly[my];
+        kept += fabs(w) < 1e-10 ? 0.0 : w * v[mx];
+      }
+    }
+  }
+  return kept;
+}
+
+// Returns the streamed population of direction al at DoF i from its departure record.
+//
+// Reference cut-off: product weights |w| < 1e-10 are dropped. When the product of the
+// smallest 1-D magnitudes is >= 1e-10 no product can fall under it and the sum factorises
+// row by row (rows through 16-, 8- or 4-byte buffer loads). Otherwise (a departure point
+// on a node line of some axis, as for DoFs on the box faces, which almost every 64-DoF wave
+// holds) the lane sums the kept terms again, one product at a time, with the source rows
+// re-read one at a time (a compact loop: the whole wave runs it when one lane needs it, so
+// it must not raise the register count of the direction loop). Lanes reflected at a wall
+// (another direction's array) gather term by term.
+template <int D, int Q, int P, typename T, int ZU>
+__device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, int al, int widx, unsigned wmask,
+                                           const DepRec<D, T>& rec) {
+  constexpr int NP = P + 1;
+  const int n0 = a.nfe[0], n1 = a.nfe[1];
+  const T* own = a.fin + (int64_t)al * a.stride;
+  const bool flip = (wmask >> al) & 1u;
+  const T* src = flip ? a.fin + (int64_t)opposite<Q>(al) * a.stride : own;
+  const double bias = widx >= 0 ? a.wall_bias[(int64_t)widx * Q + al] : 0.0;
+  if (rec.c < 0) return (T)((double)own[i] + bias);  // tracker life-time limit: identity row
+  double lx[NP], ly[NP], lz[NP];
+  const int64_t base = dep_weights<D, P, T>(a, rec, lx, ly, lz);
   double acc = 0;
-  // When no product weight can fall under the reference cut-off (product of the
-  // smallest 1-D magnitudes >= 1e-10) the sum factorises row by row; otherwise every
-  // product is formed and tested as in the assembled operator.
   double mn = 1;
   {
     double mx_ = fabs(lx[0]), my_ = fabs(ly[0]), mz_ = (D == 3) ? fabs(lz[0]) : 1.0;
@@ -247,15 +297,10 @@ __device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, i
     }
     mn = mx_ * my_ * mz_;
   }
-  if (mn >= 1e-10 && !flip) {
-    const uint64_t pb = (uint64_t)own;
-    const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(pb >> 32));
-    const uint32_t plo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)pb);
-    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)(((uint64_t)phi << 32) | (uint64_t)plo), (short)0, __builtin_amdgcn_readfirstlane(a.dir_bytes),
-        0x00020000);
+  constexpr int NZ = (D == 3) ? NP : 1;
+  if (!flip) {
+    const __amdgpu_buffer_rsrc_t rs = dir_rsrc<T>(a, own);
     const int b0 = (int)base * (int)sizeof(T);
-    constexpr int NZ = (D == 3) ? NP : 1;
 #pragma unroll 1
     for (int mz0 = 0; mz0 < NZ; mz0 += ZU) {
 #pragma unroll
@@ -275,6 +320,7 @@ __device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, i
         acc += ((D == 3) ? lz[mz] : 1.0) * accz;
       }
     }
+    if (!(mn >= 1e-10)) acc = kept_sum<D, P, T>(rs, b0, n0, n1, lx, ly, lz);
   } else if constexpr (D == 2) {
 #pragma unroll
     for (int my = 0; my <= P; my++)
@@ -309,14 +355,14 @@ __global__ __launch_bounds__((GeneralShape<D, Q, P, T, DEEP>::bs)) __attribute__
     if (widx >= 0) wmask = a.wall_flip[widx];
   }
   T fs[Q];
-  fs[0] = a.fin[i];
   using S = GeneralShape<D, Q, P, T, DEEP>;
   if constexpr (S::lds) {
+    fs[0] = a.fin[i];
     __shared__ T col[Q - 1][S::bs];
     const int t = threadIdx.x;
     if constexpr (S::prefetch) {
-      // FP64 records: the next direction's record is loaded before this direction's
-      // gathers, so its latency overlaps them (C3 FP64 370 -> 358 us per step; FP32, at its
+      // the next direction's record is loaded before this direction's gathers, so its
+      // latency overlaps them (C3 FP64 370 -> 358 us per step; FP32 at 6 waves, at its
       // 80-register cap, spilled more and lost 14%)
       DepRec<D, T> rec = load_rec<D, T>(a, i);
 #pragma unroll 1
@@ -340,6 +386,7 @@ __global__ __launch_bounds__((GeneralShape<D, Q, P, T, DEEP>::bs)) __attribute__
 #pragma unroll
     for (int al = 1; al < Q; al++)
       fs[al] = gen_direction<D, Q, P, T, S::zu>(a, i, al, widx, wmask, load_rec<D, T>(a, (int64_t)(al - 1) * a.n + i));
+    fs[0] = a.fin[i];  // not held across the unrolled loop (FP64 D3Q27 p = 3: 181 -> 143 us)
   }
   if (write_macro & STEP_STREAM_ONLY) {  // the SL stream with its wall values (CFDSolver::stream)
 #pragma unroll
@@ -373,7 +420,8 @@ static bool deep_form(const nat_ctx* ctx, int64_t n) {
 
 template <int D, int Q, int P, typename T>
 static int launch_general(nat_ctx* ctx, const GeneralArgs<T>& a, const CollideArgs<T>& ca, int wm) {
-  if (deep_form<D, Q, P, T>(ctx, a.n)) return launch_general_shape<D, Q, P, T, true>(ctx, a, ca, wm);
+  if constexpr (sizeof(T) == 4 && D == 3)
+    if (deep_form<D, Q, P, T>(ctx, a.n)) return launch_general_shape<D, Q, P, T, true>(ctx, a, ca, wm);
   return launch_general_shape<D, Q, P, T, false>(ctx, a, ca, wm);
 }
 

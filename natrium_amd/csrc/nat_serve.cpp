@@ -186,7 +186,8 @@ void initial_fields(const Req& q, const std::vector<double>& xyz, std::vector<do
   }
 }
 
-void finish(nat_server* s, Req& q);
+void release(Req& q);
+void mark_done_locked(nat_server* s, Req& q);
 
 int setup(nat_server* s, Req& q) {
   int rc = nat_create(&q.desc, &q.ctx);
@@ -227,12 +228,16 @@ void setup_worker(nat_server* s) {
       q->rc = setup(s, *q);
       if (q->rc) {
         if (q->err.empty() && q->ctx) q->err = nat_last_error(q->ctx);
-        finish(s, *q);
+        release(*q);
       }
     }
     if (q->rc) {
-      std::lock_guard<std::mutex> lk(s->m);
-      s->n_setup--;
+      {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->n_setup--;
+        mark_done_locked(s, *q);
+      }
+      s->cv_done.notify_all();
     } else {
       std::lock_guard<std::mutex> lk(s->m);
       s->ready.push_back(q);
@@ -318,7 +323,8 @@ bool issue_pack(nat_server* s, const std::vector<Req*>& qs) {
   return true;
 }
 
-void finish(nat_server* s, Req& q) {
+// GPU-side end of a request: results read back, graph, event and context released.
+void release(Req& q) {
   nat_ctx* c = q.ctx;
   if (c) {
     if (!q.rc) q.rc = nat_internal_status(c);
@@ -338,13 +344,17 @@ void finish(nat_server* s, Req& q) {
     nat_destroy(c);
     q.ctx = nullptr;
   }
-  std::lock_guard<std::mutex> lk(s->m);
+}
+
+// Book-keeping of an ended request, with s->m held by the caller in the same critical
+// section that takes the request out of its list, so that a waiter woken by state 2 sees
+// the server's counts without it.
+void mark_done_locked(nat_server* s, Req& q) {
   q.t_end = now_ms();
   q.state = 2;
   s->used -= q.bytes;
   if (!q.rc) s->node_updates += (int64_t)q.r.nsteps * q.n;
   s->busy_t1 = q.t_end;
-  s->cv_done.notify_all();
 }
 
 void run(nat_server* s) {
@@ -414,10 +424,14 @@ void run(nat_server* s) {
         progressed = true;
       }
       if (!q->in_flight && (q->remaining == 0 || q->rc)) {
-        finish(s, *q);
-        std::lock_guard<std::mutex> lk(s->m);
-        s->active[i] = s->active.back();
-        s->active.pop_back();
+        release(*q);
+        {
+          std::lock_guard<std::mutex> lk(s->m);
+          s->active[i] = s->active.back();
+          s->active.pop_back();
+          mark_done_locked(s, *q);
+        }
+        s->cv_done.notify_all();
         progressed = true;
         continue;
       }
