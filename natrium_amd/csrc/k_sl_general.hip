@@ -40,6 +40,9 @@ struct GeneralArgs {
   int nfe[3];         // FE_Q lattice nodes per axis
   int N[3];           // cells per axis
   FastDivU divN0, divN1;  // division by N[0], N[1]
+  int64_t rows;       // lattice rows along x (n / nfe[0]); 0: threads take DoFs in order
+  FastDivU divM;      // division by nfe[0] - 2, the x-interior DoFs of a row
+  FastDivU divG;      // division by 64 nfe[0], the DoFs of a group of 64 rows
   const int* cell;    // FP64 storage: [(alpha-1) * n + i] departure cell (-1: identity row)
   const double* xi;   //   unit coordinates, D per record
   const int* rec;     // FP32 storage: D + 1 int32 per record: cell * (p+1)^D + anchor node
@@ -343,11 +346,42 @@ __device__ __forceinline__ T gen_direction(const GeneralArgs<T>& a, int64_t i, i
   return (T)(acc + bias);
 }
 
+// DoF of thread g. Departure points of DoFs on the x faces of the box (x = 0 and x = L,
+// unmoved vertices) sit on node lines for the directions along those faces, and such rows
+// take the kernel's term-by-term cut-off path for the whole wave; in x-fastest order almost
+// every 64-DoF wave holds one or two of them. Threads therefore take the lattice rows in
+// groups of 64: first the x-interior DoFs of the group's rows (rows in order, DoFs in order:
+// nfe[0] - 2 whole waves), then the group's 128 x-face DoFs (two waves), so that interior
+// waves stay on the factorised path and the face waves sit beside their neighbours in the
+// grid instead of forming a tail. Loads of an interior wave's own DoFs stay contiguous
+// within each row.
+template <typename T>
+__device__ __forceinline__ int64_t dof_of(const GeneralArgs<T>& a, int64_t g) {
+  if (a.rows == 0) return g;
+  const int64_t nfe0 = a.nfe[0];
+  const uint32_t m = (uint32_t)nfe0 - 2u;
+  const uint32_t k = a.divG.div((uint32_t)g);
+  const int64_t r0 = (int64_t)k * 64;
+  const int64_t R = a.rows - r0 < 64 ? a.rows - r0 : 64;
+  const uint32_t gg = (uint32_t)(g - r0 * nfe0);
+  const uint32_t ni = (uint32_t)R * m;
+  if (gg < ni) {
+    const uint32_t q = a.divM.div(gg);
+    return (r0 + q) * nfe0 + 1 + (int64_t)(gg - q * m);
+  }
+  const uint32_t h = gg - ni;
+  return (r0 + (h >> 1)) * nfe0 + ((h & 1) ? nfe0 - 1 : 0);
+}
+
 template <int D, int Q, int P, typename T, bool DEEP>
 __global__ __launch_bounds__((GeneralShape<D, Q, P, T, DEEP>::bs)) __attribute__((amdgpu_waves_per_eu(GeneralShape<D, Q, P, T, DEEP>::waves))) void k_sl_general(
     GeneralArgs<T> a, CollideArgs<T> ca, int write_macro) {
-  const int64_t i = xcd_block_index() * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  using S = GeneralShape<D, Q, P, T, DEEP>;
+  const int64_t g = xcd_block_index() * blockDim.x + threadIdx.x;
+  if (g >= a.n) return;
+  // the grouped order where it measured faster: the LDS forms at six waves (FP32) and four
+  // (FP64); the DEEP form and the FP64 D3Q27 register form keep the lattice order
+  const int64_t i = (S::lds && !DEEP) ? dof_of(a, g) : g;
   int widx = -1;
   unsigned wmask = 0;
   if (a.wall_idx) {
@@ -355,7 +389,6 @@ __global__ __launch_bounds__((GeneralShape<D, Q, P, T, DEEP>::bs)) __attribute__
     if (widx >= 0) wmask = a.wall_flip[widx];
   }
   T fs[Q];
-  using S = GeneralShape<D, Q, P, T, DEEP>;
   if constexpr (S::lds) {
     fs[0] = a.fin[i];
     __shared__ T col[Q - 1][S::bs];
@@ -473,6 +506,12 @@ static void fill_general(nat_ctx* ctx, GeneralArgs<T>& a) {
   }
   a.divN0.set((uint32_t)a.N[0]);
   a.divN1.set((uint32_t)a.N[1]);
+  a.rows = 0;
+  if (a.nfe[0] >= 3 && a.n % a.nfe[0] == 0) {
+    a.rows = a.n / a.nfe[0];
+    a.divM.set((uint32_t)(a.nfe[0] - 2));
+    a.divG.set((uint32_t)(64 * a.nfe[0]));
+  }
   a.cell = ctx->dep_cell;
   a.xi = ctx->dep_xi64;
   a.rec = ctx->dep_rec32;
