@@ -174,8 +174,11 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
     }
     // Face neighbours this direction reads. With FP64 storage they are loaded before the
     // barrier, so that their latency overlaps the LDS exchange and the volume term
-    // (C3 SEDG FP64 0.88 -> 1.14 GLUPS); with FP32 storage that order measured 9% slower
-    // than loading them after the volume term, which is kept there.
+    // (C3 SEDG FP64 0.88 -> 1.14 GLUPS). With FP32 storage they are loaded right after
+    // the barrier, so that the volume term covers their latency: the stage then holds
+    // 110 registers (4 waves per SIMD) and C3 FP32 takes 621 us per step, against 707
+    // with the loads after the volume term (131 registers, 3 waves) and 758 before the
+    // barrier.
     constexpr bool early = sizeof(T) == 8;
     T fnb[3] = {T(0), T(0), T(0)};
     constexpr int op = opposite<Q>(al);
@@ -192,6 +195,7 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
     // in flight while the face values above are waited for
     if constexpr (al + 1 < Q) vnext = active ? s.fin[(al + 1) * s.stride + node] : T(0);
     sedg_barrier();
+    if constexpr (!early) load_faces();
     double lf = 0;
 #pragma unroll
     for (int b = 0; b < D; b++) {
@@ -201,7 +205,6 @@ __global__ __launch_bounds__((SedgShape<D, P>::NT)) void k_sedg_stage(SedgArgs<T
       for (int j = 0; j < NP; j++) der = fma(dl[mi[b]][j], buf[al & 1][base + j * lstr[b]], der);
       lf = fma(-G[b], der, lf);
     }
-    if constexpr (!early) load_faces();
 #pragma unroll
     for (int a = 0; a < D; a++) {
       const double c = fac[a] * G[a];
