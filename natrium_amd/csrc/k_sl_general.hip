@@ -73,8 +73,9 @@ struct GeneralArgs {
 // the FP32 3D kernel runs at 4 waves with those planes in flight and the next
 // direction's record prefetched; it is launched when the grid fits one round of it,
 // where each wave's dependent loads, not the number of waves, set the time (D3Q27 p = 3
-// on 16^3 perturbed cells, 460 blocks: 156 -> 138 us per step). A grid that fills the
-// 6-wave form (C3: 1520 blocks) keeps that form (the 4-wave one: 205 -> 245 us).
+// on 16^3 perturbed cells, 460 blocks: 219 us per step in the 6-wave form, 118 in the
+// DEEP one). A grid that fills the 6-wave form (C3: 1520 blocks) keeps that form (179 us
+// against 194 for the 4-wave one).
 template <int D, int Q, int P, typename T, bool DEEP>
 struct GeneralShape {
   static constexpr bool lds = sizeof(T) == 4 || Q * 8 <= 160;
